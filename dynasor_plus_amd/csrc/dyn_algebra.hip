@@ -1,0 +1,881 @@
+// CP-ALS algebra on the device and the CP-ALS driver.
+//
+//   Gram        G_n = Y_n^T Y_n from FP32 factors on MFMA (v_mfma_f32_32x32x2f32
+//               / 16x16x4f32), FP32 per-workgroup partials summed in FP64 in a
+//               fixed order (la::gram, src/dynasor_linalg.cpp:47-99).
+//   Cholesky    V_n = ones .* G_m (m != n) in FP64, ridge * max(diag) added,
+//               right-looking factorisation with the reference's defensive
+//               zero pivots (src/dynasor_linalg.cpp:104-200,
+//               src/dynasor_cpals.cpp:292-305).
+//   Solve       X = Yhat V^{-1} by per-row forward/back substitution in FP64
+//               (src/dynasor_linalg.cpp:210-246); column sums of x^2 in FP64.
+//   Normalise   lambda_r = ||X[:, r]||_2, Y = X / lambda (zero when lambda
+//               <= 1e-20) (src/dynasor_linalg.cpp:251-316); the same pass
+//               refreshes the BF16 shadow and the Gram partials, and for the
+//               last mode the <Y, Yhat> column partials of the fit.
+//   Fit         ||T||^2 - 2 <T, M> + ||M||^2 in FP64 with the Yhat identity
+//               (Gauss-Seidel) or a forward-model pass (Jacobi)
+//               (src/dynasor_cpals.cpp:79-113, 332-373).
+#include "dyn_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace dyn {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kMaxGramWG = 1024;
+constexpr int kSolveBlock = 256;
+constexpr int kMaxSolveWG = 1024;
+
+// ---------------------------------------------------------------------------
+//  FP64 reduction of ||vals||^2
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sumsq(const float* __restrict__ v, uint64_t n,
+                                                  double* __restrict__ part) {
+    __shared__ double red[kBlock / 64];
+    double s = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock) {
+        const double x = v[i];
+        s += x * x;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_d(const double* __restrict__ part, uint32_t n, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t = 0.0;
+    for (uint32_t i = 0; i < n; ++i) t += part[i];
+    *out = t;
+}
+
+int tensor_norm2(dyn_gpu_ctx* c, double* out) {
+    if (c->tnorm2 >= 0.0) {
+        *out = c->tnorm2;
+        return DYN_OK;
+    }
+    const unsigned nb = 1024;
+    void* sp = nullptr;
+    DYN_CHECK(scratch2(c, (nb + 8) * sizeof(double), &sp));
+    double* part = (double*)sp;
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, c->stream, c->vals, c->nnz, part);
+    hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(64), 0, c->stream, part, nb, part + nb);
+    DYN_CHECK(check_launch(c, "sumsq"));
+    double h = 0.0;
+    DYN_HIP(c, hipMemcpyAsync(&h, part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    c->tnorm2 = h;
+    *out = h;
+    return DYN_OK;
+}
+
+// ---------------------------------------------------------------------------
+//  Scale + Gram pass.  Rows are consumed in MFMA operand order: with the
+//  32x32x2 instruction lane l holds row (l / 32) of a row pair at column
+//  (l % 32) of a 32-column block (A = Y^T and B = Y take the same value);
+//  with 16x16x4, row (l / 16) of a row quad at column (l % 16).
+// ---------------------------------------------------------------------------
+struct ScaleArgs {
+    float*        Y;          // rows x RP, scaled in place when inv != null
+    bf16*         Ybf;        // BF16 shadow to refresh (nullable)
+    const double* inv;        // 1 / lambda per column (nullable: Gram only)
+    const float*  Yhat;       // non-null: accumulate <Y, Yhat> per column
+    float*        gpart;      // [nwg][RP][RP]
+    double*       ipart;      // [nwg][RP]
+    uint32_t      rows;
+    uint32_t      nwg;
+};
+
+__device__ __forceinline__ bf16 to_bf16(float x) { return __float2bfloat16(x); }
+
+template <int RP>
+__global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
+    __shared__ float tile[RP * RP];
+    __shared__ double ired[kBlock / 64][RP];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    constexpr int W = kBlock / 64;
+
+    if constexpr (RP == 16) {
+        // 16x16x4: one tile; each wave step consumes 4 rows
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        const int col = lane & 15;
+        const int rsub = lane >> 4;
+        const double inv = a.inv ? a.inv[col] : 1.0;
+        double ip = 0.0;
+        for (uint32_t r0 = (blockIdx.x * W + wave) * 4u; r0 < a.rows; r0 += a.nwg * W * 4u) {
+            const uint32_t r = r0 + rsub;
+            float y = 0.f;
+            if (r < a.rows) {
+                y = a.Y[(size_t)r * RP + col];
+                if (a.inv) {
+                    y = (float)((double)y * inv);
+                    a.Y[(size_t)r * RP + col] = y;
+                }
+                if (a.Ybf) a.Ybf[(size_t)r * RP + col] = to_bf16(y);
+                if (a.Yhat) ip += (double)y * (double)a.Yhat[(size_t)r * RP + col];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y, y, acc, 0, 0, 0);
+        }
+        // deterministic cross-wave sum through LDS (wave order)
+        for (int w = 0; w < W; ++w) {
+            if (wave == w) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = 4 * (lane >> 4) + v;
+                    const int j = lane & 15;
+                    if (w == 0) tile[i * RP + j] = acc[v];
+                    else tile[i * RP + j] += acc[v];
+                }
+            }
+            __syncthreads();
+        }
+        if (a.Yhat) {
+            // lanes with the same column: l, l+16, l+32, l+48
+            ip += __shfl_xor(ip, 16, 64);
+            ip += __shfl_xor(ip, 32, 64);
+            if (lane < 16) ired[wave][lane] = ip;
+        }
+    } else {
+        constexpr int TB = RP / 32;                       // 32-column blocks
+        constexpr bool kSplitRows = (TB <= 2);            // else: wave w owns block-row w
+        constexpr int NT = kSplitRows ? TB * TB : TB;     // tiles per wave
+        v16f acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+        const int col = lane & 31;
+        const int rsub = lane >> 5;
+        double inv[TB];
+#pragma unroll
+        for (int b = 0; b < TB; ++b) inv[b] = a.inv ? a.inv[b * 32 + col] : 1.0;
+        double ip[TB];
+#pragma unroll
+        for (int b = 0; b < TB; ++b) ip[b] = 0.0;
+        const uint32_t step = kSplitRows ? (uint32_t)W * 2u : 2u;
+        const uint32_t first = kSplitRows ? (blockIdx.x * W + wave) * 2u : blockIdx.x * 2u;
+        const uint32_t stride = kSplitRows ? a.nwg * W * 2u : a.nwg * 2u;
+        (void)step;
+        for (uint32_t r0 = first; r0 < a.rows; r0 += stride) {
+            const uint32_t r = r0 + rsub;
+            float y[TB];
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                y[b] = 0.f;
+                if (r < a.rows) {
+                    const size_t off = (size_t)r * RP + b * 32 + col;
+                    float x = a.Y[off];
+                    if (a.inv) x = (float)((double)x * inv[b]);
+                    y[b] = x;
+                    // in the block-row split every wave reads the whole row;
+                    // wave b alone writes column block b
+                    const bool writer = kSplitRows || wave == b;
+                    if (writer) {
+                        if (a.inv) a.Y[off] = x;
+                        if (a.Ybf) a.Ybf[off] = to_bf16(x);
+                        if (a.Yhat) ip[b] += (double)x * (double)a.Yhat[off];
+                    }
+                }
+            }
+            if constexpr (kSplitRows) {
+#pragma unroll
+                for (int bm = 0; bm < TB; ++bm)
+#pragma unroll
+                    for (int bn = 0; bn < TB; ++bn)
+                        acc[bm * TB + bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                            y[bm], y[bn], acc[bm * TB + bn], 0, 0, 0);
+            } else {
+                float ym = y[0];
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+                    if (b == wave) ym = y[b];
+#pragma unroll
+                for (int bn = 0; bn < TB; ++bn)
+                    acc[bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(ym, y[bn], acc[bn], 0, 0, 0);
+            }
+        }
+        // accumulator layout (32x32): register v of lane l holds
+        // C[8*(v/4) + 4*(l/32) + v%4][l%32]
+        if constexpr (kSplitRows) {
+            for (int w = 0; w < W; ++w) {
+                if (wave == w) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int bm = t / TB, bn = t % TB;
+#pragma unroll
+                        for (int v = 0; v < 16; ++v) {
+                            const int i = bm * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+                            const int j = bn * 32 + (lane & 31);
+                            if (w == 0) tile[i * RP + j] = acc[t][v];
+                            else tile[i * RP + j] += acc[t][v];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+#pragma unroll
+            for (int bn = 0; bn < NT; ++bn)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = wave * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+                    const int j = bn * 32 + (lane & 31);
+                    tile[i * RP + j] = acc[bn][v];
+                }
+            __syncthreads();
+        }
+        if (a.Yhat) {
+#pragma unroll
+            for (int b = 0; b < TB; ++b) {
+                double t = ip[b];
+                t += __shfl_xor(t, 32, 64);
+                if (lane < 32) {
+                    if (kSplitRows) ired[wave][b * 32 + lane] = t;
+                    else if (wave == b) ired[0][b * 32 + lane] = t;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float* gp = a.gpart + (size_t)blockIdx.x * RP * RP;
+    for (int i = tid; i < RP * RP; i += kBlock) gp[i] = tile[i];
+    if (a.Yhat && tid < RP) {
+        double t = 0.0;
+        const bool split = (RP == 16) || (RP / 32 <= 2);
+        if (split) {
+            for (int w = 0; w < W; ++w) t += ired[w][tid];
+        } else {
+            t = ired[0][tid];
+        }
+        a.ipart[(size_t)blockIdx.x * RP + tid] = t;
+    }
+}
+
+template <int RP>
+__global__ void k_gram_reduce(const float* __restrict__ gpart, uint32_t nwg,
+                              double* __restrict__ G) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= RP * RP) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nwg; ++b) s += (double)gpart[(size_t)b * RP * RP + e];
+    G[e] = s;
+}
+
+// ---------------------------------------------------------------------------
+//  V = ones .* G_m (m != n), ridge, Cholesky, then V^{-1} (one workgroup,
+//  FP64 in LDS).
+//
+//  The reference solves every row with a forward and a backward substitution
+//  whose zero pivots yield zero components (src/dynasor_linalg.cpp:210-246).
+//  Both substitutions are linear maps: with S the set of non-zero pivots the
+//  forward one is M = embed(L_SS^{-1}) and the backward one is M^T, so the
+//  row update is x = yhat . (M^T M).  The workgroup forms M column by column
+//  (forward substitution on unit vectors under the same zero-pivot rule) and
+//  then W = M^T M; the row pass is a GEMV against W.
+// ---------------------------------------------------------------------------
+struct CholArgs {
+    const double* G[kMaxModes];
+    int           N;
+    int           n;
+    int           R;
+    double        ridge;
+    double*       W;          // RP x RP out: V^{-1} (zeros outside R x R)
+};
+
+template <int RP>
+__global__ __launch_bounds__(kBlock) void k_chol(CholArgs a) {
+    __shared__ double V[RP * RP];     // lower: L ; strict upper: M^T (V[j][i] = M[i][j])
+    __shared__ double Md[RP];         // diag of M
+    __shared__ double s_inv;
+    __shared__ double s_dmax;
+    const int tid = threadIdx.x;
+    const int R = a.R;
+    for (int e = tid; e < RP * RP; e += kBlock) {
+        const int r = e / RP, c = e % RP;
+        double v = 0.0;
+        if (r < R && c < R) {
+            v = 1.0;
+            for (int m = 0; m < a.N; ++m) {
+                if (m == a.n) continue;
+                v *= a.G[m][e];
+            }
+        }
+        V[e] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double dmax = 0.0;
+        for (int r = 0; r < R; ++r) dmax = fmax(dmax, V[r * RP + r]);
+        s_dmax = dmax;
+    }
+    __syncthreads();
+    const double eps = fmax(a.ridge * s_dmax, 0.0);
+    if (tid < R) V[tid * RP + tid] += eps;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            double s = V[j * RP + j];
+            for (int k = 0; k < j; ++k) s -= V[j * RP + k] * V[j * RP + k];
+            if (s < 0.0) s = 0.0;
+            const double d = sqrt(s);
+            V[j * RP + j] = d;
+            s_inv = (d > 0.0) ? 1.0 / d : 0.0;
+        }
+        __syncthreads();
+        const double inv = s_inv;
+        for (int i = j + 1 + tid; i < R; i += kBlock) {
+            double t = V[i * RP + j];
+            for (int k = 0; k < j; ++k) t -= V[i * RP + k] * V[j * RP + k];
+            V[i * RP + j] = t * inv;
+        }
+        __syncthreads();
+    }
+    // column j of M: L m = e_j with zero pivots -> zero components
+    for (int j = tid; j < R; j += kBlock) {
+        const double dj = V[j * RP + j];
+        const double mj = (dj > 0.0) ? 1.0 / dj : 0.0;
+        Md[j] = mj;
+        for (int i = j + 1; i < R; ++i) {
+            double s = -V[i * RP + j] * mj;
+            for (int k = j + 1; k < i; ++k) s -= V[i * RP + k] * V[j * RP + k];
+            const double di = V[i * RP + i];
+            V[j * RP + i] = (di > 0.0) ? s / di : 0.0;
+        }
+    }
+    __syncthreads();
+    // W[r][c] = sum_{i >= max(r, c)} M[i][r] M[i][c]
+    for (int e = tid; e < RP * RP; e += kBlock) {
+        const int r = e / RP, c = e % RP;
+        double w = 0.0;
+        if (r < R && c < R) {
+            for (int i = (r > c ? r : c); i < R; ++i) {
+                const double mr = (i == r) ? Md[r] : V[r * RP + i];
+                const double mc = (i == c) ? Md[c] : V[c * RP + i];
+                w += mr * mc;
+            }
+        }
+        a.W[e] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+//  X = Yhat W per row (FP64 GEMV, W = V^{-1} in LDS), column sums of x^2 per
+//  workgroup in FP64.
+// ---------------------------------------------------------------------------
+template <int RP>
+__global__ __launch_bounds__(kSolveBlock) void k_solve(const float* __restrict__ Yhat,
+                                                       float* __restrict__ X,
+                                                       const double* __restrict__ Wg,
+                                                       uint32_t rows,
+                                                       double* __restrict__ sq_part) {
+    __shared__ double W[RP * RP];
+    __shared__ double red[kSolveBlock / 64][RP];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    for (int e = tid; e < RP * RP; e += kSolveBlock) W[e] = Wg[e];
+    __syncthreads();
+    constexpr int NA = (RP + 63) / 64;
+    double sq[NA];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) sq[j] = 0.0;
+
+    const uint32_t stride = gridDim.x * kSolveBlock;
+    // every thread of a wave runs the same number of trips (wave-uniform
+    // bound) so the cross-lane reductions below see all 64 lanes
+    const uint32_t base0 = blockIdx.x * kSolveBlock + (uint32_t)(wave * 64);
+    for (uint32_t b = base0; b < rows; b += stride) {
+        const uint32_t i = b + lane;
+        const bool ok = i < rows;
+        double x[RP];
+#pragma unroll
+        for (int c = 0; c < RP; ++c) x[c] = 0.0;
+        const float* yrow = Yhat + (size_t)(ok ? i : 0) * RP;
+        for (int r4 = 0; r4 < RP / 4; ++r4) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok) v = *reinterpret_cast<const float4*>(yrow + r4 * 4);
+            const double yv[4] = {(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double* wr = W + (r4 * 4 + k) * RP;
+#pragma unroll
+                for (int c = 0; c < RP; ++c) x[c] = fma(yv[k], wr[c], x[c]);
+            }
+        }
+        if (ok) {
+#pragma unroll
+            for (int c4 = 0; c4 < RP / 4; ++c4) {
+                float4 v = make_float4((float)x[c4 * 4 + 0], (float)x[c4 * 4 + 1],
+                                       (float)x[c4 * 4 + 2], (float)x[c4 * 4 + 3]);
+                *reinterpret_cast<float4*>(X + (size_t)i * RP + c4 * 4) = v;
+            }
+        }
+        // column sums of x^2 across the wave; lane (c % 64) keeps column c
+#pragma unroll
+        for (int c = 0; c < RP; ++c) {
+            double t = ok ? x[c] * x[c] : 0.0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (lane == (c & 63)) sq[c >> 6] += t;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+        const int c = j * 64 + lane;
+        if (c < RP) red[wave][c] = sq[j];
+    }
+    __syncthreads();
+    if (tid < RP) {
+        double t = 0.0;
+        for (int w = 0; w < kSolveBlock / 64; ++w) t += red[w][tid];
+        sq_part[(size_t)blockIdx.x * RP + tid] = t;
+    }
+}
+
+__global__ void k_lambda(const double* __restrict__ sq_part, uint32_t nwg, int R, int RP,
+                         double* __restrict__ lambda, double* __restrict__ inv) {
+    const int c = threadIdx.x;
+    if (c >= RP) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nwg; ++b) s += sq_part[(size_t)b * RP + c];
+    const double l = (c < R) ? sqrt(s) : 0.0;
+    lambda[c] = l;
+    inv[c] = (l > 1e-20) ? 1.0 / l : 0.0;
+}
+
+// fit: out[0] = fit, [1] = residual, [2] = <T,M>, [3] = ||M||^2
+struct FitArgs {
+    const double* G[kMaxModes];
+    int           N;
+    int           R;
+    int           RP;
+    const double* lambda;
+    const double* ipart;      // [nwg][RP] <Y, Yhat> column partials, or null
+    uint32_t      nwg;
+    const double* inner_dev;  // used when ipart is null (forward-model dot)
+    double        tnorm2;
+    double*       out;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
+    __shared__ double red[kBlock];
+    __shared__ double s_inner;
+    const int tid = threadIdx.x;
+    // <T, M>
+    double part = 0.0;
+    if (a.ipart) {
+        for (int c = tid; c < a.R; c += kBlock) {
+            double s = 0.0;
+            for (uint32_t b = 0; b < a.nwg; ++b) s += a.ipart[(size_t)b * a.RP + c];
+            part += a.lambda[c] * s;
+        }
+    }
+    red[tid] = part;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kBlock; ++i) t += red[i];
+        s_inner = a.ipart ? t : *a.inner_dev;
+    }
+    __syncthreads();
+    // ||M||^2 = sum_r lambda_r sum_c lambda_c prod_n G_n[r, c]
+    double rs = 0.0;
+    for (int r = tid; r < a.R; r += kBlock) {
+        double rowsum = 0.0;
+        for (int c = 0; c < a.R; ++c) {
+            double h = 1.0;
+            for (int n = 0; n < a.N; ++n) h *= a.G[n][(size_t)r * a.RP + c];
+            rowsum += a.lambda[c] * h;
+        }
+        rs += a.lambda[r] * rowsum;
+    }
+    red[tid] = rs;
+    __syncthreads();
+    if (tid == 0) {
+        double mn = 0.0;
+        for (int i = 0; i < kBlock; ++i) mn += red[i];
+        mn = fmax(mn, 0.0);
+        const double inner = s_inner;
+        double r2 = a.tnorm2 - 2.0 * inner + mn;
+        if (r2 < 0.0) r2 = 0.0;
+        const double resid = sqrt(r2);
+        const double tn = sqrt(a.tnorm2);
+        a.out[0] = 1.0 - resid / fmax(tn, 1e-30);
+        a.out[1] = resid;
+        a.out[2] = inner;
+        a.out[3] = mn;
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ y, bf16* __restrict__ o, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) o[i] = to_bf16(y[i]);
+}
+
+int refresh_bf16(dyn_gpu_ctx* c, int mode) {
+    const size_t n = (size_t)c->dims[mode] * c->Rp;
+    const unsigned nb = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_to_bf16, dim3(nb), dim3(kBlock), 0, c->stream, c->Y[mode],
+                       c->Ybf[mode], n);
+    return check_launch(c, "to_bf16");
+}
+
+// ---------------------------------------------------------------------------
+//  host wrappers
+// ---------------------------------------------------------------------------
+static uint32_t gram_wgs(uint32_t rows, int RP) {
+    const uint32_t rows_per_step = (RP == 16) ? 16u : (RP <= 64 ? 8u : 2u);
+    uint32_t want = (rows + rows_per_step * 16 - 1) / (rows_per_step * 16);
+    return std::max<uint32_t>(1, std::min<uint32_t>(want, kMaxGramWG));
+}
+
+template <int RP>
+static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const double* inv, bool inner, bool bf,
+                         double** ipart_out, uint32_t* nwg_out) {
+    ScaleArgs a;
+    a.Y = c->Y[mode];
+    a.Ybf = bf ? c->Ybf[mode] : nullptr;
+    a.inv = inv;
+    a.Yhat = inner ? c->Yhat[mode] : nullptr;
+    a.rows = c->dims[mode];
+    a.nwg = gram_wgs(a.rows, RP);
+    void* sp = nullptr;
+    DYN_CHECK(scratch2(c, (size_t)a.nwg * RP * RP * sizeof(float) +
+                              (size_t)a.nwg * RP * sizeof(double) + 256,
+                       &sp));
+    a.ipart = (double*)sp;
+    a.gpart = (float*)((char*)sp + (size_t)a.nwg * RP * sizeof(double));
+    hipLaunchKernelGGL(k_scale_gram<RP>, dim3(a.nwg), dim3(kBlock), 0, c->stream, a);
+    DYN_CHECK(check_launch(c, "scale_gram"));
+    hipLaunchKernelGGL(k_gram_reduce<RP>, dim3((RP * RP + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       c->stream, a.gpart, a.nwg, c->G[mode]);
+    DYN_CHECK(check_launch(c, "gram_reduce"));
+    if (ipart_out) *ipart_out = a.ipart;
+    if (nwg_out) *nwg_out = a.nwg;
+    return DYN_OK;
+}
+
+static int scale_gram(dyn_gpu_ctx* c, int mode, const double* inv, bool inner, bool bf,
+                      double** ipart_out, uint32_t* nwg_out) {
+    switch (c->Rp) {
+        case 16: return scale_gram_rp<16>(c, mode, inv, inner, bf, ipart_out, nwg_out);
+        case 32: return scale_gram_rp<32>(c, mode, inv, inner, bf, ipart_out, nwg_out);
+        case 64: return scale_gram_rp<64>(c, mode, inv, inner, bf, ipart_out, nwg_out);
+        case 128: return scale_gram_rp<128>(c, mode, inv, inner, bf, ipart_out, nwg_out);
+        default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
+    }
+}
+
+int gram_mode(dyn_gpu_ctx* c, int mode) {
+    return scale_gram(c, mode, nullptr, false, false, nullptr, nullptr);
+}
+
+static int chol(dyn_gpu_ctx* c, int n, double ridge) {
+    CholArgs a;
+    for (int m = 0; m < c->N; ++m) a.G[m] = c->G[m];
+    a.N = c->N;
+    a.n = n;
+    a.R = c->R;
+    a.ridge = ridge;
+    a.W = c->L;
+    switch (c->Rp) {
+        case 16: hipLaunchKernelGGL(k_chol<16>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
+        case 32: hipLaunchKernelGGL(k_chol<32>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
+        case 64: hipLaunchKernelGGL(k_chol<64>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
+        case 128: hipLaunchKernelGGL(k_chol<128>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
+        default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
+    }
+    return check_launch(c, "chol");
+}
+
+static int solve_and_lambda(dyn_gpu_ctx* c, int n) {
+    const uint32_t rows = c->dims[n];
+    const uint32_t nwg = std::max<uint32_t>(
+        1, std::min<uint32_t>((rows + kSolveBlock - 1) / kSolveBlock, kMaxSolveWG));
+    void* sp = nullptr;
+    DYN_CHECK(scratch(c, (size_t)nwg * c->Rp * sizeof(double), &sp));
+    double* part = (double*)sp;
+    switch (c->Rp) {
+        case 16: hipLaunchKernelGGL(k_solve<16>, dim3(nwg), dim3(kSolveBlock), 0, c->stream, c->Yhat[n], c->Y[n], c->L, rows, part); break;
+        case 32: hipLaunchKernelGGL(k_solve<32>, dim3(nwg), dim3(kSolveBlock), 0, c->stream, c->Yhat[n], c->Y[n], c->L, rows, part); break;
+        case 64: hipLaunchKernelGGL(k_solve<64>, dim3(nwg), dim3(kSolveBlock), 0, c->stream, c->Yhat[n], c->Y[n], c->L, rows, part); break;
+        case 128: hipLaunchKernelGGL(k_solve<128>, dim3(nwg), dim3(kSolveBlock), 0, c->stream, c->Yhat[n], c->Y[n], c->L, rows, part); break;
+        default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
+    }
+    DYN_CHECK(check_launch(c, "solve"));
+    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(kMaxRp), 0, c->stream, part, nwg, c->R, c->Rp,
+                       c->lambda, c->inv_lambda);
+    return check_launch(c, "lambda");
+}
+
+static int fit_eval(dyn_gpu_ctx* c, const double* ipart, uint32_t nwg, const double* inner_dev,
+                    double tnorm2) {
+    FitArgs a;
+    for (int m = 0; m < c->N; ++m) a.G[m] = c->G[m];
+    a.N = c->N;
+    a.R = c->R;
+    a.RP = c->Rp;
+    a.lambda = c->lambda;
+    a.ipart = ipart;
+    a.nwg = nwg;
+    a.inner_dev = inner_dev;
+    a.tnorm2 = tnorm2;
+    a.out = c->fitbuf;
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(kBlock), 0, c->stream, a);
+    return check_launch(c, "fit");
+}
+
+}  // namespace dyn
+
+using namespace dyn;
+
+// ===========================================================================
+//  C-ABI: CP-ALS
+// ===========================================================================
+extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result* res) {
+    if (!c || !o || !res) return DYN_ERR_INVALID;
+    if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    if (o->kernel != DYN_CP_GAUSS_SEIDEL && o->kernel != DYN_CP_JACOBI)
+        return set_err(c, DYN_ERR_INVALID, "bad kernel %d", o->kernel);
+    if (o->precision != DYN_PREC_FP32 && o->precision != DYN_PREC_BF16)
+        return set_err(c, DYN_ERR_INVALID, "bad precision %d", o->precision);
+    if (o->max_iters < 0) return set_err(c, DYN_ERR_INVALID, "max_iters < 0");
+    DYN_HIP(c, hipSetDevice(c->device));
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double>(b - a).count();
+    };
+    const int N = c->N;
+    const bool jacobi = o->kernel == DYN_CP_JACOBI;
+    const bool bf = o->precision == DYN_PREC_BF16;
+
+    double* hist = res->fit_history;
+    double* itim = res->iter_time_s;
+    std::memset(res, 0, sizeof(*res));
+    res->fit_history = hist;
+    res->iter_time_s = itim;
+
+    auto t_r0 = clk::now();
+    for (int m = 0; m < N; ++m)
+        if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    res->remap_time_s = secs(t_r0, clk::now());
+
+    double tn2 = 0.0;
+    DYN_CHECK(tensor_norm2(c, &tn2));
+    res->tensor_norm = std::sqrt(tn2);
+
+    // initial Grams, lambda = 1, BF16 shadows
+    for (int m = 0; m < N; ++m) DYN_CHECK(gram_mode(c, m));
+    {
+        std::vector<double> ones(c->Rp, 0.0);
+        for (int r = 0; r < c->R; ++r) ones[r] = 1.0;
+        DYN_HIP(c, hipMemcpyAsync(c->lambda, ones.data(), c->Rp * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+    }
+    if (bf)
+        for (int m = 0; m < N; ++m) DYN_CHECK(refresh_bf16(c, m));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+
+    if (o->verbose) {
+        std::printf("[cpals] modes=%d  rank=%d  padded=%d  nnz=%llu  ||T|| = %.6e  "
+                    "max_iters=%d  tol=%.1e  kernel=%s  precision=%s\n",
+                    N, c->R, c->Rp, (unsigned long long)c->nnz, res->tensor_norm, o->max_iters,
+                    o->tol, jacobi ? "jacobi" : "gauss-seidel", bf ? "bf16-gather" : "fp32");
+    }
+
+    hipEvent_t ev[2 * kMaxModes];
+    for (int i = 0; i < 2 * kMaxModes; ++i) DYN_HIP(c, hipEventCreate(&ev[i]));
+    auto destroy_events = [&]() {
+        for (int i = 0; i < 2 * kMaxModes; ++i) (void)hipEventDestroy(ev[i]);
+    };
+
+    int rc = DYN_OK;
+    double fit_prev = 0.0;
+    auto t_start = clk::now();
+    double* inner_dev = c->fitbuf + 8;
+    for (int iter = 0; iter < o->max_iters && rc == DYN_OK; ++iter) {
+        auto t0 = clk::now();
+        const void* Yin[kMaxModes];
+        for (int m = 0; m < N; ++m) Yin[m] = bf ? (const void*)c->Ybf[m] : (const void*)c->Y[m];
+        int nev = 0;
+        if (jacobi) {
+            (void)hipEventRecord(ev[0], c->stream);
+            for (int m = 0; m < N && rc == DYN_OK; ++m)
+                rc = mttkrp_mode(c, m, o->precision, Yin, nullptr, c->Yhat[m], c->Rp);
+            (void)hipEventRecord(ev[1], c->stream);
+            nev = 1;
+        }
+        double* ipart = nullptr;
+        uint32_t inwg = 0;
+        for (int n = 0; n < N && rc == DYN_OK; ++n) {
+            if (!jacobi) {
+                (void)hipEventRecord(ev[2 * n], c->stream);
+                rc = mttkrp_mode(c, n, o->precision, Yin, nullptr, c->Yhat[n], c->Rp);
+                (void)hipEventRecord(ev[2 * n + 1], c->stream);
+                nev = n + 1;
+                if (rc != DYN_OK) break;
+            }
+            if ((rc = chol(c, n, o->ridge)) != DYN_OK) break;
+            if ((rc = solve_and_lambda(c, n)) != DYN_OK) break;
+            const bool last = (n == N - 1) && !jacobi;
+            rc = scale_gram(c, n, c->inv_lambda, last, bf, last ? &ipart : nullptr,
+                            last ? &inwg : nullptr);
+        }
+        if (rc != DYN_OK) break;
+        if (jacobi) {
+            // true <T, M> from the forward model at the stored coordinates
+            float* Ys[kMaxModes];
+            for (int m = 0; m < N; ++m) Ys[m] = c->Y[m];
+            rc = forward_eval(c, Ys, c->Rp, c->lambda, nullptr, c->vals, inner_dev);
+            if (rc != DYN_OK) break;
+        }
+        if ((rc = fit_eval(c, ipart, inwg, inner_dev, tn2)) != DYN_OK) break;
+        hipError_t e = hipMemcpyAsync(c->fitbuf_host, c->fitbuf, 4 * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = set_err(c, DYN_ERR_HIP, "cpals iteration %d failed: %s", iter,
+                         hipGetErrorString(e));
+            break;
+        }
+        double mt = 0.0;
+        for (int i = 0; i < nev; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) mt += ms * 1e-3;
+        }
+        const double fit = c->fitbuf_host[0];
+        const double resid = c->fitbuf_host[1];
+        const double it = secs(t0, clk::now());
+        if (res->fit_history) res->fit_history[iter] = fit;
+        if (res->iter_time_s) res->iter_time_s[iter] = it;
+        res->final_fit = fit;
+        res->final_residual = resid;
+        res->iters_done = iter + 1;
+        res->mttkrp_time_s += mt;
+        if (o->verbose) {
+            std::printf("[cpals] iter %3d  fit = %.6f  dfit = %.2e  resid = %.3e  iter = %.3f s\n",
+                        iter, fit, iter == 0 ? 0.0 : std::fabs(fit - fit_prev), resid, it);
+        }
+        if (iter > 0 && std::fabs(fit - fit_prev) < o->tol) {
+            if (o->verbose)
+                std::printf("[cpals] converged after %d iterations (|dfit|=%.2e < %.1e)\n",
+                            iter + 1, std::fabs(fit - fit_prev), o->tol);
+            break;
+        }
+        fit_prev = fit;
+    }
+    destroy_events();
+    res->total_time_s = secs(t_start, clk::now());
+    res->linalg_time_s = std::max(0.0, res->total_time_s - res->mttkrp_time_s);
+    c->bf16_valid = bf;
+    if (rc == DYN_OK && o->verbose) {
+        std::printf("[cpals] done  iters=%d  final_fit=%.6f  resid=%.3e\n", res->iters_done,
+                    res->final_fit, res->final_residual);
+    }
+    return rc;
+}
+
+// ===========================================================================
+//  C-ABI: forward / adjoint / completion
+// ===========================================================================
+static bool rp_ok(int ld) { return ld == 16 || ld == 32 || ld == 64 || ld == 128; }
+
+extern "C" int dyn_gpu_forward(dyn_gpu_ctx* c, int use_lambda, float* m_out) {
+    if (!c || !m_out) return DYN_ERR_INVALID;
+    if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    DYN_HIP(c, hipSetDevice(c->device));
+    if (c->nnz == 0) return DYN_OK;
+    void* sp = nullptr;
+    DYN_CHECK(scratch(c, c->nnz * sizeof(float), &sp));
+    float* Ys[kMaxModes];
+    for (int m = 0; m < c->N; ++m) Ys[m] = c->Y[m];
+    DYN_CHECK(forward_eval(c, Ys, c->Rp, use_lambda ? c->lambda : nullptr, (float*)sp, nullptr,
+                           nullptr));
+    DYN_HIP(c, hipMemcpyAsync(m_out, sp, c->nnz * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
+extern "C" int dyn_gpu_adjoint(dyn_gpu_ctx* c, const float* g, int mode, float* grad_out,
+                               int ld) {
+    if (!c || !g || !grad_out) return DYN_ERR_INVALID;
+    if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
+    if (ld < c->R) return set_err(c, DYN_ERR_INVALID, "ld %d < rank %d", ld, c->R);
+    DYN_HIP(c, hipSetDevice(c->device));
+    if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
+    const size_t rows = c->dims[mode];
+    void* sp = nullptr;
+    DYN_CHECK(scratch(c, std::max<uint64_t>(c->nnz, 1) * sizeof(float) +
+                             rows * c->Rp * sizeof(float) + 256,
+                      &sp));
+    float* gd = (float*)sp;
+    float* od = (float*)((char*)sp + ((std::max<uint64_t>(c->nnz, 1) * sizeof(float) + 255) & ~size_t(255)));
+    if (c->nnz)
+        DYN_HIP(c, hipMemcpyAsync(gd, g, c->nnz * sizeof(float), hipMemcpyHostToDevice,
+                                  c->stream));
+    const void* Yin[kMaxModes];
+    for (int m = 0; m < c->N; ++m) Yin[m] = c->Y[m];
+    DYN_CHECK(mttkrp_mode(c, mode, DYN_PREC_FP32, Yin, gd, od, c->Rp));
+    std::vector<float> pad(rows * c->Rp);
+    DYN_HIP(c, hipMemcpyAsync(pad.data(), od, pad.size() * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < rows; ++i)
+        std::memcpy(grad_out + i * (size_t)ld, &pad[i * c->Rp], sizeof(float) * c->R);
+    return DYN_OK;
+}
+
+extern "C" int dyn_gpu_forward_dev(dyn_gpu_ctx* c, const float* const* A, int ld, float* m) {
+    if (!c || !A || !m) return DYN_ERR_INVALID;
+    if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
+    DYN_HIP(c, hipSetDevice(c->device));
+    if (c->nnz == 0) return DYN_OK;
+    return forward_eval(c, A, ld, nullptr, m, nullptr, nullptr);
+}
+
+extern "C" int dyn_gpu_adjoint_dev(dyn_gpu_ctx* c, const float* g, int mode,
+                                   const float* const* A, int ld, float* grad) {
+    if (!c || !g || !A || !grad) return DYN_ERR_INVALID;
+    if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
+    if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
+    DYN_HIP(c, hipSetDevice(c->device));
+    if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
+    const void* Yin[kMaxModes];
+    for (int m = 0; m < c->N; ++m) Yin[m] = A[m];
+    return mttkrp_mode(c, mode, DYN_PREC_FP32, Yin, g, grad, ld);
+}
+
+extern "C" int dyn_gpu_completion_grad_dev(dyn_gpu_ctx* c, int loss, const float* const* A,
+                                           int ld, float* const* grad, double* loss_out) {
+    if (!c || !A || !grad) return DYN_ERR_INVALID;
+    if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    if (loss != DYN_LOSS_LS && loss != DYN_LOSS_BERNOULLI)
+        return set_err(c, DYN_ERR_INVALID, "bad loss %d", loss);
+    if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
+    DYN_HIP(c, hipSetDevice(c->device));
+    for (int m = 0; m < c->N; ++m)
+        if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
+    return completion_grad(c, loss, A, ld, grad, loss_out);
+}

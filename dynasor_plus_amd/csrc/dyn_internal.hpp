@@ -1,0 +1,156 @@
+// Internal state of a dyn_gpu_ctx and helpers shared by the HIP translation
+// units of the engine.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/dynasor_gpu.h"
+
+namespace dyn {
+
+constexpr int kMaxModes = DYN_GPU_MAX_MODES;
+constexpr int kBlock = 256;          // threads per workgroup for the streaming kernels
+constexpr int kSeg = 64;             // nonzeros per lane group in the segmented spMTTKRP
+constexpr int kMaxRp = DYN_GPU_MAX_RANK;
+
+typedef __hip_bfloat16 bf16;
+
+// Smallest power of two >= max(R, 16): the device row stride of every factor.
+inline int padded_rank(int R) {
+    int p = 16;
+    while (p < R) p <<= 1;
+    return p;
+}
+
+// One mode's sorted copy: the nonzeros ordered by idx[mode] (ties in input
+// order, or by a secondary key), with that mode's index column dropped and
+// replaced by a row pointer.
+struct ModeCopy {
+    bool      built = false;
+    int       secondary = -1;
+    float*    vals = nullptr;                   // nnz
+    uint32_t* idx[kMaxModes] = {nullptr};      // idx[w] for w != mode
+    uint32_t* rowptr = nullptr;                 // I_mode + 1
+    uint32_t* seg = nullptr;                    // ceil(nnz / kSeg) + 1 : row holding position s*kSeg
+    uint32_t* perm = nullptr;                   // nnz : sorted position -> input position
+};
+
+struct TimedPair {
+    hipEvent_t a, b;
+};
+
+}  // namespace dyn
+
+struct dyn_gpu_ctx {
+    int          device = 0;
+    hipStream_t  own_stream = nullptr;
+    hipStream_t  stream = nullptr;
+    std::string  err;
+
+    // memory accounting
+    std::unordered_map<void*, size_t> allocs;
+    uint64_t     mem_used = 0;
+    uint64_t     mem_cap = 0;      // 0 = unlimited
+
+    // tensor (input order)
+    int          N = 0;
+    uint64_t     nnz = 0;
+    uint32_t     dims[dyn::kMaxModes] = {0};
+    float*       vals = nullptr;
+    uint32_t*    idx[dyn::kMaxModes] = {nullptr};
+    double       tnorm2 = -1.0;    // ||T||^2 (FP64), computed lazily
+    dyn::ModeCopy copy[dyn::kMaxModes];
+
+    // factors
+    int          R = 0;
+    int          Rp = 0;
+    float*       Y[dyn::kMaxModes] = {nullptr};
+    dyn::bf16*   Ybf[dyn::kMaxModes] = {nullptr};
+    float*       Yhat[dyn::kMaxModes] = {nullptr};
+    double*      G[dyn::kMaxModes] = {nullptr};   // Rp x Rp FP64 Grams
+    double*      lambda = nullptr;                // Rp
+    double*      inv_lambda = nullptr;            // Rp
+    double*      L = nullptr;                     // Rp x Rp inverse of the Hadamard Gram
+    double*      fitbuf = nullptr;                // device scalars of the fit
+    double*      fitbuf_host = nullptr;           // pinned mirror
+    bool         bf16_valid = false;              // Ybf mirrors Y
+
+    // scratch (grown on demand)
+    void*        scratch = nullptr;
+    size_t       scratch_bytes = 0;
+    void*        scratch2 = nullptr;
+    size_t       scratch2_bytes = 0;
+
+    // timing of spMTTKRP launches
+    bool         timing = false;
+    std::vector<dyn::TimedPair> timed;
+    std::vector<dyn::TimedPair> event_pool;
+};
+
+namespace dyn {
+
+// error helpers -------------------------------------------------------------
+int set_err(dyn_gpu_ctx* c, int code, const char* fmt, ...);
+
+#define DYN_HIP(ctx, call)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::dyn::set_err((ctx), DYN_ERR_HIP, "%s failed: %s (%s:%d)", \
+                                  #call, hipGetErrorString(e_), __FILE__,     \
+                                  __LINE__);                                  \
+    } while (0)
+
+#define DYN_CHECK(expr)                 \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != DYN_OK) return rc_;  \
+    } while (0)
+
+// device memory owned by the context ---------------------------------------
+int dalloc(dyn_gpu_ctx* c, void** p, size_t bytes);
+void dfree(dyn_gpu_ctx* c, void* p);
+template <typename T>
+inline int dalloc_t(dyn_gpu_ctx* c, T** p, size_t count) {
+    return dalloc(c, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+// scratch buffer of at least `bytes` (contents not preserved across calls)
+int scratch(dyn_gpu_ctx* c, size_t bytes, void** p);
+int scratch2(dyn_gpu_ctx* c, size_t bytes, void** p);
+
+int check_launch(dyn_gpu_ctx* c, const char* what);
+
+// building blocks implemented in the .hip units -----------------------------
+int remap_mode(dyn_gpu_ctx* c, int mode, int secondary);
+void free_copy(dyn_gpu_ctx* c, int mode);
+
+// Segmented spMTTKRP of mode `mode` from the per-mode copy into `out`
+// (I_mode x Rp floats, device).  `Yin[w]` are the gathered factors (FP32 or
+// BF16 according to `prec`; entry `mode` ignored), `vals_override` replaces
+// the copy's values with vals_override[perm[k]] when non-null.
+int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
+                const float* vals_override, float* out, int Rp);
+
+int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp,
+                 const double* lambda_or_null, float* m_out, const float* vals_dot,
+                 double* dot_out_dev);
+int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
+                    float* const* grad, double* loss_out);
+
+int tensor_norm2(dyn_gpu_ctx* c, double* out);
+int gram_mode(dyn_gpu_ctx* c, int mode);
+int refresh_bf16(dyn_gpu_ctx* c, int mode);
+int fill_f32(dyn_gpu_ctx* c, float* p, size_t n, float v);
+
+void timing_begin(dyn_gpu_ctx* c, TimedPair* tp);
+void timing_end(dyn_gpu_ctx* c, TimedPair* tp);
+
+}  // namespace dyn

@@ -1,0 +1,207 @@
+/*
+ * dynasor_gpu.h -- C-ABI of the MI355X sparse CP-ALS engine.
+ *
+ * Plain C types only (pointers, sizes, fixed-width integers); no HIP or
+ * torch types cross this boundary.  Every entry point returns an int status
+ * (DYN_OK == 0) and never calls exit(); the last error text of a context is
+ * available from dyn_gpu_last_error().
+ *
+ * Conventions
+ *   - One context = one device + one HIP stream.  Calls on a context are
+ *     serialised by the caller (one host thread); contexts are not shared
+ *     across threads.
+ *   - Host buffers are caller-owned and copied; device buffers are owned by
+ *     the context.  Entry points with a `_dev` suffix take device pointers
+ *     and run asynchronously on the context stream (see dyn_gpu_set_stream).
+ *   - Factor matrices on the host are row-major I_n x R with a leading
+ *     dimension `ld >= R` (the reference pads rows to Rp, include/
+ *     dynasor_common.h:604-615 and src/dynasor_io.cpp:157-160, so callers can
+ *     pass their padded buffers unchanged).
+ *   - Indices are 0-based uint32 (include/dynasor_common.h:28), values are
+ *     float32 (include/dynasor_common.h:30).  nnz must be < 2^32.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   dyn_gpu_upload_coo   <- Tensor SoA slabs filled by load_coo_tensor
+ *                           (include/dynasor_common.h:322-600,
+ *                            src/dynasor_io.cpp:1184-1261)
+ *   dyn_gpu_remap        <- build_flycoo + NCopy fiber sort + CSR rowptr
+ *                           (src/dynasor_preprocess.cpp:2013-2163, 1609-2008,
+ *                            1467-1607)
+ *   dyn_gpu_mttkrp       <- SpmttkrpSweep::process / dynasor_process_shard_
+ *                           fiber_csr and the JIT C-ABI dyn_jit_fiber_csr
+ *                           (src/dynasor.cpp:352-539, src/dynasor_kernel.cpp:
+ *                            597-710, include/dynasor_jit.h:50-57)
+ *   dyn_gpu_mttkrp_all   <- spmttkrp_all_modes_dynasor
+ *                           (src/dynasor.cpp:592-817, src/dynasor_kernel.cpp:
+ *                            830-1039)
+ *   dyn_gpu_gram         <- la::gram (src/dynasor_linalg.cpp:47-99)
+ *   dyn_gpu_cpals        <- cpals() (include/dynasor_cpals.h:121,
+ *                           src/dynasor_cpals.cpp:120-396)
+ *   dyn_gpu_forward / dyn_gpu_adjoint: no reference counterpart (sparse CP
+ *                           completion); the adjoint is the reference MTTKRP
+ *                           (src/dynasor_reference.cpp:18-50) applied to the
+ *                           loss-gradient tensor.
+ */
+#ifndef DYNASOR_GPU_H
+#define DYNASOR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DYN_GPU_ABI_VERSION 1
+#define DYN_GPU_MAX_MODES   8      /* DYN_MAX_MODES, include/dynasor_common.h:24 */
+#define DYN_GPU_MAX_RANK    128    /* padded rank held in LDS by the R x R kernels */
+
+/* status codes */
+#define DYN_OK                 0
+#define DYN_ERR_INVALID        1   /* bad argument (sizes, mode, rank, null) */
+#define DYN_ERR_STATE          2   /* call order (e.g. mttkrp before remap) */
+#define DYN_ERR_OOM            3   /* device allocation failed / over cap */
+#define DYN_ERR_HIP            4   /* HIP runtime error */
+#define DYN_ERR_UNSUPPORTED    5   /* shape outside the compiled grid */
+
+/* gathered-operand precision of the spMTTKRP family */
+#define DYN_PREC_FP32          0   /* FP32 factors, FP32 accumulate */
+#define DYN_PREC_BF16          1   /* BF16 factor shadow gathered, FP32 accumulate */
+
+/* CP-ALS sweep order (CpKernel, include/dynasor_cpals.h:77-80) */
+#define DYN_CP_GAUSS_SEIDEL    0
+#define DYN_CP_JACOBI          1
+
+/* sparse-completion losses */
+#define DYN_LOSS_LS            0   /* 0.5 (m - y)^2 */
+#define DYN_LOSS_BERNOULLI     1   /* log(1 + e^m) - y m */
+
+typedef struct dyn_gpu_ctx dyn_gpu_ctx;
+
+/* ---- context --------------------------------------------------------- */
+/* Create a context on `device`.  Environment DYN_GPU_MEM_CAP_MB (optional)
+ * caps the context's device allocations so tests can force DYN_ERR_OOM. */
+int         dyn_gpu_create(int device, dyn_gpu_ctx** out);
+int         dyn_gpu_destroy(dyn_gpu_ctx* ctx);
+const char* dyn_gpu_last_error(const dyn_gpu_ctx* ctx);
+int         dyn_gpu_abi_version(void);
+/* Run subsequent work on `stream` (a hipStream_t, or NULL for the
+ * context's own stream). */
+int         dyn_gpu_set_stream(dyn_gpu_ctx* ctx, void* stream);
+int         dyn_gpu_synchronize(dyn_gpu_ctx* ctx);
+/* Bytes of device memory currently held by the context. */
+int         dyn_gpu_mem_used(const dyn_gpu_ctx* ctx, uint64_t* bytes);
+
+/* ---- tensor ---------------------------------------------------------- */
+/* Copy a COO tensor (SoA: vals[nnz], idx[n][nnz]) to the device.  Replaces
+ * any previous tensor and drops per-mode copies and factors.  Indices must
+ * satisfy idx[n][k] < dims[n] (checked on the device; DYN_ERR_INVALID). */
+int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
+                       const uint32_t* dims, const float* vals,
+                       const uint32_t* const* idx);
+
+/* Build the mode-`mode` copy: a stable device radix sort of the nonzeros by
+ * idx[mode] (ties keep input order), the gathered value / other-index
+ * columns, the row pointer (rowptr[r]..rowptr[r+1] = nonzeros of row r) and
+ * the equal-nnz chunk -> first-row segment table.  Optional host outputs:
+ * rowptr_out[dims[mode]+1], perm_out[nnz] (sorted position -> input
+ * position).  `secondary` >= 0 additionally orders each row by
+ * idx[secondary] (composite key); -1 keeps input order within rows. */
+int dyn_gpu_remap(dyn_gpu_ctx* ctx, int mode, int secondary,
+                  uint64_t* rowptr_out, uint32_t* perm_out);
+/* Build every mode's copy (secondary key: the first other mode). */
+int dyn_gpu_remap_all(dyn_gpu_ctx* ctx);
+/* Copy the mode-`mode` sorted copy back: vals_out[nnz] and, for every mode w
+ * != mode, idx_out[w][nnz] (idx_out[mode] is ignored and may be NULL). */
+int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
+                     uint32_t* const* idx_out);
+
+/* ---- factors --------------------------------------------------------- */
+/* Allocate factor storage for rank R (device rows padded to a power of two
+ * >= 16) and copy Y[n] (I_n x R, leading dimension ld) in; Y may be NULL to
+ * only allocate (factors zeroed). */
+int dyn_gpu_set_factors(dyn_gpu_ctx* ctx, int rank, const float* const* Y,
+                        int ld);
+int dyn_gpu_get_factors(dyn_gpu_ctx* ctx, int mode, float* Y, int ld);
+int dyn_gpu_get_lambda(dyn_gpu_ctx* ctx, double* lambda);
+
+/* ---- kernels --------------------------------------------------------- */
+/* Yhat_mode = MTTKRP of the current factors (definition:
+ * src/dynasor_reference.cpp:18-50).  Yhat_out (host, I_mode x R, ld) may be
+ * NULL to leave the result on the device only. */
+int dyn_gpu_mttkrp(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_out,
+                   int ld);
+/* Every Yhat_n from one snapshot of the factors in one tensor pass
+ * (Jacobi all-modes).  Yhat_out may be NULL, or an array of N host
+ * pointers (entries may be NULL). */
+int dyn_gpu_mttkrp_all(dyn_gpu_ctx* ctx, int precision, float* const* Yhat_out,
+                       int ld);
+/* G = Y_mode^T Y_mode (R x R, row-major, FP64 result of FP32 MFMA partials). */
+int dyn_gpu_gram(dyn_gpu_ctx* ctx, int mode, double* G);
+
+/* ---- CP-ALS ---------------------------------------------------------- */
+typedef struct dyn_cp_opts {
+    int      max_iters;     /* CpOptions::max_iters */
+    double   tol;           /* stop when |fit_i - fit_{i-1}| < tol (i > 0) */
+    double   ridge;         /* relative Cholesky ridge, CpOptions::ridge */
+    int      kernel;        /* DYN_CP_GAUSS_SEIDEL | DYN_CP_JACOBI */
+    int      precision;     /* DYN_PREC_FP32 | DYN_PREC_BF16 */
+    int      verbose;       /* print "[cpals] iter ..." lines to stdout */
+    int      reserved[5];
+} dyn_cp_opts;
+
+typedef struct dyn_cp_result {
+    int      iters_done;
+    double   final_fit;
+    double   final_residual;
+    double   tensor_norm;
+    double   total_time_s;
+    double   mttkrp_time_s;  /* sum of spMTTKRP kernel time (HIP events) */
+    double   linalg_time_s;  /* the rest of the iteration */
+    double   remap_time_s;   /* one-time per-mode copies built inside cpals */
+    /* caller-provided arrays of length >= max_iters, or NULL */
+    double*  fit_history;
+    double*  iter_time_s;
+} dyn_cp_result;
+
+/* Run CP-ALS from the factors currently set (dyn_gpu_set_factors).  Builds
+ * missing per-mode copies first.  On return the device factors are column-
+ * normalised and lambda holds the column weights (dyn_gpu_get_lambda). */
+int dyn_gpu_cpals(dyn_gpu_ctx* ctx, const dyn_cp_opts* opts,
+                  dyn_cp_result* res);
+
+/* ---- sparse completion: forward / adjoint ----------------------------- */
+/* m[k] = sum_r lambda_r prod_n A_n[idx_n[k], r] at every stored coordinate,
+ * in upload order, from the current factors (lambda: the context's lambda if
+ * use_lambda, else ones).  m_out: host nnz floats. */
+int dyn_gpu_forward(dyn_gpu_ctx* ctx, int use_lambda, float* m_out);
+/* grad_out (host, I_mode x R, ld) = MTTKRP_mode of the sparse tensor with
+ * values g[k] (upload order) at the stored coordinates, from the current
+ * factors: the adjoint of dyn_gpu_forward with respect to A_mode. */
+int dyn_gpu_adjoint(dyn_gpu_ctx* ctx, const float* g, int mode,
+                    float* grad_out, int ld);
+
+/* Device-pointer variants used by the autograd harness; they read factors
+ * from A[n] (device, I_n x ld floats) rather than the context's factors. */
+int dyn_gpu_forward_dev(dyn_gpu_ctx* ctx, const float* const* A, int ld,
+                        float* m);
+int dyn_gpu_adjoint_dev(dyn_gpu_ctx* ctx, const float* g, int mode,
+                        const float* const* A, int ld, float* grad);
+/* Fused loss + gradient for every mode over the per-mode copies, with the
+ * stored tensor values as labels y: loss = sum_k l(m_k, y_k) and
+ * grad[n] = MTTKRP_n(l'(m, y)).  A, grad: device, I_n x ld floats. */
+int dyn_gpu_completion_grad_dev(dyn_gpu_ctx* ctx, int loss, const float* const* A,
+                                int ld, float* const* grad, double* loss_out);
+
+/* ---- timing ---------------------------------------------------------- */
+/* When enabled, every spMTTKRP launch is bracketed by HIP events on the
+ * launch stream; dyn_gpu_kernel_stats returns (and resets) the number of
+ * launches and their summed duration in milliseconds. */
+int dyn_gpu_enable_timing(dyn_gpu_ctx* ctx, int on);
+int dyn_gpu_kernel_stats(dyn_gpu_ctx* ctx, uint64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DYNASOR_GPU_H */

@@ -1,0 +1,260 @@
+"""Device engine vs the oracle and the reference's golden vectors.
+
+Integer/index work (remap permutation, row pointer, copies) must be bit-exact;
+FP32 spMTTKRP must agree with the reference --verify outputs within 1e-5
+Frobenius-relative (the reference's own check is 1e-4, src/main.cpp:955-968);
+the plain CP-ALS fit must track the reference per iteration within 1e-6
+(plus half a unit of the printed %.6f).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from dynasor_plus_amd import GpuContext, Tensor
+from dynasor_plus_amd import _lib as L
+from tests._oracle import tensor_from_npz
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+VERIFY = ["verify_3d_7_r8.npz", "verify_3d_7_r16.npz", "verify_3d_7_r32.npz",
+          "verify_3d_7_r64.npz", "verify_synth_cp_r5_r8.npz", "verify_g4_r16.npz",
+          "verify_g5_r32.npz"]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def frob_rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def skewed_tensor(rng, dims, nnz, heavy_frac=0.4, val_kind="normal"):
+    """Distinct coordinates; a `heavy_frac` share of the nonzeros in row 0 of
+    every mode, so rows span lane groups, workgroups and the fix-up pass."""
+    N = len(dims)
+    cells = int(np.prod(dims, dtype=np.float64))
+    nh = int(nnz * heavy_frac)
+    lin = set()
+    out = []
+    sub = dims[1:]
+    while len(out) < nh:
+        c = (0,) + tuple(int(rng.integers(0, d)) for d in sub)
+        k = np.ravel_multi_index(c, dims)
+        if k not in lin:
+            lin.add(k)
+            out.append(k)
+    while len(out) < nnz:
+        k = int(rng.integers(0, cells))
+        if k not in lin:
+            lin.add(k)
+            out.append(k)
+    out = np.array(out, dtype=np.int64)
+    rng.shuffle(out)
+    idx = [i.astype(np.uint32) for i in np.unravel_index(out, dims)]
+    vals = rng.standard_normal(nnz).astype(np.float32)
+    return Tensor(tuple(dims), vals, idx)
+
+
+@pytest.mark.parametrize("name", VERIFY)
+def test_mttkrp_matches_reference_verify(gpu_ctx, oracle, name):
+    say("verify", name)
+    z = golden(name)
+    T = tensor_from_npz(z)
+    R = int(z["R"])
+    Y = oracle.init(T.dims, R, int(z["seed"]), kind="uniform")
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, R)
+    for n in range(T.num_modes):
+        got = gpu_ctx.mttkrp(n)
+        err = frob_rel(got, z[f"yhat{n}"])
+        assert err <= 1e-5, (name, n, err)
+
+
+@pytest.mark.parametrize("dims,nnz,R", [
+    ((40, 30, 20), 5000, 5), ((64, 48, 32), 20000, 16), ((300, 200, 100), 60000, 32),
+    ((50, 40, 30, 20), 30000, 64), ((30, 20, 16, 12, 8), 20000, 32), ((200, 150), 9000, 20),
+    ((20, 16, 12, 10, 8, 6), 15000, 8), ((100, 80, 60), 40000, 100)])
+def test_mttkrp_vs_oracle_skewed(gpu_ctx, oracle, dims, nnz, R):
+    say("mttkrp skewed", dims, nnz, R)
+    rng = np.random.default_rng(nnz + R)
+    T = skewed_tensor(rng, dims, nnz)
+    Y = [rng.standard_normal((d, R)).astype(np.float32) for d in dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, R)
+    for n in range(len(dims)):
+        want = oracle.mttkrp_f64(T, Y, n)
+        got = gpu_ctx.mttkrp(n)
+        assert frob_rel(got, want) <= 1e-5, (n, frob_rel(got, want))
+
+
+def test_mttkrp_bf16_operands(gpu_ctx, oracle):
+    say("mttkrp bf16")
+    rng = np.random.default_rng(11)
+    T = skewed_tensor(rng, (120, 90, 70), 30000)
+    R = 32
+    Y = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, R)
+
+    def to_bf16(a):   # round-to-nearest-even to bfloat16, back to float
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        return (u.astype(np.uint32) << 16).view(np.float32)
+
+    Yb = [to_bf16(y) for y in Y]
+    for n in range(3):
+        got = gpu_ctx.mttkrp(n, L.DYN_PREC_BF16)
+        assert frob_rel(got, oracle.mttkrp_f64(T, Yb, n)) <= 1e-5
+        assert frob_rel(got, oracle.mttkrp_f64(T, Y, n)) <= 2e-2
+
+
+def test_mttkrp_bitwise_reproducible(gpu_ctx):
+    say("mttkrp reproducible")
+    rng = np.random.default_rng(12)
+    T = skewed_tensor(rng, (80, 70, 60), 50000, heavy_frac=0.6)
+    Y = [rng.standard_normal((d, 32)).astype(np.float32) for d in T.dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, 32)
+    a = [gpu_ctx.mttkrp(n) for n in range(3)]
+    b = [gpu_ctx.mttkrp(n) for n in range(3)]
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("secondary", [-1, 1])
+def test_remap_bit_exact(gpu_ctx, oracle, secondary):
+    say("remap", secondary)
+    rng = np.random.default_rng(13)
+    T = skewed_tensor(rng, (300, 257, 70000), 40000)
+    gpu_ctx.upload(T)
+    for mode in range(3):
+        sec = secondary if secondary != mode else (mode + 1) % 3
+        rowptr, perm = gpu_ctx.remap(mode, sec)
+        operm, orow = oracle.remap(T, mode, sec)
+        np.testing.assert_array_equal(perm, operm)
+        np.testing.assert_array_equal(rowptr, orow)
+        vals, cols = gpu_ctx.get_copy(mode)
+        np.testing.assert_array_equal(vals, T.vals[operm])
+        for w, c in cols.items():
+            np.testing.assert_array_equal(c, T.idx[w][operm])
+
+
+def test_gram_mfma(gpu_ctx):
+    say("gram")
+    rng = np.random.default_rng(14)
+    for R, dims in ((16, (1000, 37, 5)), (32, (5000, 3, 64)), (64, (3001, 65, 7)),
+                    (128, (2000, 130, 9)), (7, (100, 50, 20))):
+        T = skewed_tensor(rng, dims, 200, heavy_frac=0.0)
+        Y = [rng.standard_normal((d, R)).astype(np.float32) for d in dims]
+        gpu_ctx.upload(T)
+        gpu_ctx.set_factors(Y, R)
+        for n in range(3):
+            Yd = Y[n].astype(np.float64)
+            want = Yd.T @ Yd
+            got = gpu_ctx.gram(n)
+            assert frob_rel(got, want) < 1e-6, (R, n, frob_rel(got, want))
+
+
+def test_cpals_c1_tracks_reference(gpu_ctx):
+    say("cpals c1")
+    from tests.test_oracle import c1_tensor
+    T, z = c1_tensor()
+    Y0 = [z[f"init{n}"] for n in range(3)]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y0, 16)
+    res = gpu_ctx.run_cpals(int(z["iters"]), float(z["tol"]))
+    ref = z["fits"]
+    assert res.iters_done == len(ref)
+    np.testing.assert_allclose(res.fit_history, ref, rtol=0, atol=1.5e-6)
+    Y = gpu_ctx.get_factors()
+    for n in range(3):
+        assert frob_rel(Y[n], z[f"final{n}"]) < 1e-3
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_cpals_vs_fp64_oracle(gpu_ctx, oracle, jacobi):
+    say("cpals fp64", jacobi)
+    rng = np.random.default_rng(15)
+    T = skewed_tensor(rng, (60, 50, 40, 30), 30000, heavy_frac=0.2)
+    R = 8
+    Y0 = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
+    fits, Yo, lam = oracle.cpals(T, Y0, 8, 0.0, jacobi=jacobi, fp64=True)
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y0, R)
+    res = gpu_ctx.run_cpals(8, 0.0, kernel=int(jacobi))
+    np.testing.assert_allclose(res.fit_history, fits, rtol=0, atol=1e-5)
+    np.testing.assert_allclose(res.lambda_, lam, rtol=1e-3)
+
+
+def test_forward_and_adjoint(gpu_ctx, oracle):
+    say("forward/adjoint")
+    rng = np.random.default_rng(16)
+    T = skewed_tensor(rng, (70, 60, 50), 20000)
+    R = 24
+    A = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(A, R)
+    m = gpu_ctx.forward(use_lambda=False)
+    want = oracle.forward(T, A)
+    assert frob_rel(m, want) < 1e-5
+    g = rng.standard_normal(T.nnz).astype(np.float32)
+    Tg = Tensor(T.dims, g, T.idx)
+    for n in range(3):
+        got = gpu_ctx.adjoint(g, n)
+        assert frob_rel(got, oracle.mttkrp_f64(Tg, A, n)) < 1e-5
+
+
+def test_edge_cases(gpu_ctx, oracle):
+    say("edges")
+    rng = np.random.default_rng(17)
+    # mode of size 1, empty rows, duplicates summed (MTTKRP is linear)
+    idx = [np.array([0, 0, 0, 5, 5], dtype=np.uint32), np.array([0, 0, 0, 0, 0], dtype=np.uint32),
+           np.array([1, 1, 2, 0, 3], dtype=np.uint32)]
+    T = Tensor((9, 1, 4), np.array([1, 2, 3, 4, 5], dtype=np.float32), idx)
+    Y = [rng.standard_normal((d, 3)).astype(np.float32) for d in T.dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, 3)
+    for n in range(3):
+        assert frob_rel(gpu_ctx.mttkrp(n), oracle.mttkrp_f64(T, Y, n)) < 1e-6
+    # no nonzeros at all
+    T0 = Tensor((3, 4), np.zeros(0, dtype=np.float32),
+                [np.zeros(0, dtype=np.uint32), np.zeros(0, dtype=np.uint32)])
+    gpu_ctx.upload(T0)
+    gpu_ctx.set_factors([np.ones((3, 2), np.float32), np.ones((4, 2), np.float32)], 2)
+    assert not gpu_ctx.mttkrp(0).any()
+
+
+def test_error_paths(gpu_ctx):
+    say("errors")
+    T = Tensor((3, 3), np.ones(2, np.float32), [np.array([0, 5], np.uint32),
+                                                np.array([1, 2], np.uint32)])
+    with pytest.raises(L.DynGpuError) as e:
+        gpu_ctx.upload(T)
+    assert e.value.code == L.DYN_ERR_INVALID
+    T = Tensor((3, 3), np.ones(2, np.float32), [np.array([0, 2], np.uint32),
+                                                np.array([1, 2], np.uint32)])
+    gpu_ctx.upload(T)
+    with pytest.raises(L.DynGpuError) as e:
+        gpu_ctx.set_factors(None, 200)
+    assert e.value.code == L.DYN_ERR_UNSUPPORTED
+    with pytest.raises(L.DynGpuError) as e:
+        gpu_ctx.mttkrp(0)
+    assert e.value.code == L.DYN_ERR_STATE
+
+
+def test_memory_cap_reports_oom(monkeypatch):
+    say("oom cap")
+    monkeypatch.setenv("DYN_GPU_MEM_CAP_MB", "1")
+    rng = np.random.default_rng(18)
+    T = skewed_tensor(rng, (1000, 1000, 1000), 200000, heavy_frac=0.0)
+    with GpuContext(0) as ctx:
+        with pytest.raises(L.DynGpuError) as e:
+            ctx.upload(T)
+        assert e.value.code == L.DYN_ERR_OOM

@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted.
+
+Each oracle function is checked against something it does not share code
+with: the reference binary's own outputs (tests/golden/, produced by
+tests/golden/make_golden.py from oracle/_ref), closed forms, numpy dense
+algebra, or invariants that a wrong index / sign / dropped term breaks.
+"""
+import numpy as np
+import pytest
+
+from tests._oracle import tensor_from_npz
+from tests.conftest import golden
+from dynasor_plus_amd.tensor_io import Tensor
+
+VERIFY = ["verify_3d_7_r8.npz", "verify_3d_7_r16.npz", "verify_3d_7_r32.npz",
+          "verify_3d_7_r64.npz", "verify_synth_cp_r5_r8.npz", "verify_g4_r16.npz",
+          "verify_g5_r32.npz"]
+
+
+def frob_rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def dense(T):
+    D = np.zeros(T.dims, dtype=np.float64)
+    np.add.at(D, tuple(c.astype(np.int64) for c in T.idx), T.vals.astype(np.float64))
+    return D
+
+
+def rand_tensor(rng, dims, nnz):
+    cells = int(np.prod(dims))
+    lin = rng.choice(cells, size=nnz, replace=False)
+    idx = list(np.unravel_index(lin, dims))
+    vals = rng.standard_normal(nnz).astype(np.float32)
+    return Tensor(dims, vals, [i.astype(np.uint32) for i in idx])
+
+
+def mttkrp_dense(D, Y, n):
+    """Dense MTTKRP by einsum: unfold-free contraction over every mode != n."""
+    N = D.ndim
+    letters = "abcdefgh"[:N]
+    ops = [D]
+    subs = [letters]
+    for w in range(N):
+        if w != n:
+            ops.append(np.asarray(Y[w], dtype=np.float64))
+            subs.append(letters[w] + "r")
+    return np.einsum(",".join(subs) + "->" + letters[n] + "r", *ops)
+
+
+@pytest.mark.parametrize("name", VERIFY)
+def test_mttkrp_matches_reference_verify(oracle, name):
+    z = golden(name)
+    T = tensor_from_npz(z)
+    R = int(z["R"])
+    Y = oracle.init(T.dims, R, int(z["seed"]), kind="uniform")
+    for n in range(T.num_modes):
+        ref = z[f"yhat{n}"]
+        got = oracle.mttkrp(T, Y, n)
+        assert got.shape == ref.shape
+        assert frob_rel(got, ref) <= 1e-6, (name, n, frob_rel(got, ref))
+
+
+def test_mttkrp_dense_definition(oracle):
+    rng = np.random.default_rng(3)
+    for dims in [(5, 4, 3), (4, 3, 5, 2), (3, 2, 4, 2, 3)]:
+        T = rand_tensor(rng, dims, int(np.prod(dims)) // 2)
+        Y = [rng.standard_normal((d, 7)).astype(np.float32) for d in dims]
+        D = dense(T)
+        for n in range(len(dims)):
+            assert frob_rel(oracle.mttkrp(T, Y, n), mttkrp_dense(D, Y, n)) < 1e-6
+            assert frob_rel(oracle.mttkrp_f64(T, Y, n), mttkrp_dense(D, Y, n)) < 1e-13
+
+
+def test_mttkrp_rank_one_closed_form(oracle):
+    rng = np.random.default_rng(4)
+    a, b, c = rng.random(6), rng.random(5), rng.random(4)
+    D = np.einsum("i,j,k->ijk", a, b, c)
+    idx = [i.astype(np.uint32).ravel() for i in np.indices(D.shape)]
+    T = Tensor(D.shape, D.ravel().astype(np.float32), idx)
+    Y = [rng.standard_normal((d, 3)) for d in D.shape]
+    want = np.outer(a, (b @ Y[1]) * (c @ Y[2]))
+    got = oracle.mttkrp_f64(T, Y, 0)
+    assert frob_rel(got, want) < 1e-6     # T itself is stored in float32
+
+
+def test_mttkrp_column_invariant(oracle):
+    """sum_i Yhat_n[i, r] Y_n[i, r] = <T, [[Y]]_r> for every mode n."""
+    rng = np.random.default_rng(5)
+    T = rand_tensor(rng, (9, 7, 8, 6), 200)
+    Y = [rng.standard_normal((d, 5)) for d in T.dims]
+    cols = [np.einsum("ir,ir->r", oracle.mttkrp_f64(T, Y, n), Y[n]) for n in range(4)]
+    for n in range(1, 4):
+        np.testing.assert_allclose(cols[n], cols[0], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["cpals_synth_cp_r5_gauss-seidel_r5.npz",
+                                  "cpals_synth_cp_r5_jacobi_r5.npz",
+                                  "cpals_3d_7_gauss-seidel_r2.npz",
+                                  "cpals_c1_gauss-seidel_r16.npz"])
+def test_init_normal_matches_reference_dump(oracle, name):
+    z = golden(name)
+    dims = tuple(int(z[f"init{n}"].shape[0]) for n in range(8) if f"init{n}" in z)
+    Y = oracle.init(dims, int(z["R"]), int(z["seed"]), kind="normal")
+    for n in range(len(dims)):
+        np.testing.assert_array_equal(Y[n], z[f"init{n}"])
+
+
+def c1_tensor():
+    from dynasor_plus_amd import host
+    z = golden("cpals_c1_gauss-seidel_r16.npz")
+    T, _, _ = host.generate(tuple(int(d) for d in z["gen_dims"]), int(z["gen_nnz"]),
+                            seed=int(z["gen_seed"]), coords="uniform", values="planted",
+                            planted_rank=int(z["gen_rank"]), noise_frac=float(z["gen_noise"]))
+    return T, z
+
+
+def test_c1_generator_is_stable():
+    import hashlib
+    T, z = c1_tensor()
+    h = hashlib.sha256()
+    h.update(np.array(T.dims, dtype=np.uint32).tobytes())
+    h.update(T.vals.tobytes())
+    for c in T.idx:
+        h.update(c.tobytes())
+    assert h.hexdigest() == str(z["checksum"])
+
+
+@pytest.mark.parametrize("name", ["cpals_c1_gauss-seidel_r16.npz",
+                                  "cpals_3d_7_gauss-seidel_r2.npz"])
+def test_cpals_fp32_matches_reference_fits(oracle, name):
+    z = golden(name)
+    T = c1_tensor()[0] if "c1" in name else tensor_from_npz(z)
+    N = T.num_modes
+    Y0 = [z[f"init{n}"] for n in range(N)]
+    fits, Y, lam = oracle.cpals(T, Y0, int(z["iters"]), float(z["tol"]))
+    ref = z["fits"]
+    assert len(fits) == len(ref)
+    # the reference prints %.6f: half a unit of the last digit plus 1e-6
+    np.testing.assert_allclose(fits, ref, rtol=0, atol=1.5e-6)
+    for n in range(N):
+        assert frob_rel(Y[n], z[f"final{n}"]) < 1e-4
+    np.testing.assert_allclose(lam, z["lam"], rtol=1e-4)
+
+
+def test_cpals_synth_cp_r5_recovers_planted_model(oracle):
+    z = golden("cpals_synth_cp_r5_gauss-seidel_r5.npz")
+    T = tensor_from_npz(z)
+    Y0 = [z[f"init{n}"] for n in range(3)]
+    fits, _, _ = oracle.cpals(T, Y0, int(z["iters"]), float(z["tol"]))
+    ref = z["fits"]
+    # away from the FP32 cancellation regime both agree to the printed digits
+    k = min(len(fits), len(ref), 6)
+    np.testing.assert_allclose(fits[:k], ref[:k], rtol=0, atol=2e-6)
+    fits64, _, _ = oracle.cpals(T, Y0, 30, 1e-9, fp64=True)
+    assert fits64[-1] > 0.9999
+
+
+def test_cpals_fit_equals_dense_reconstruction(oracle):
+    rng = np.random.default_rng(7)
+    T = rand_tensor(rng, (7, 6, 5), 120)
+    Y0 = [rng.standard_normal((d, 3)).astype(np.float32) for d in T.dims]
+    for jacobi in (False, True):
+        fits, Y, lam = oracle.cpals(T, Y0, 5, 0.0, fp64=True, jacobi=jacobi)
+        M = np.einsum("r,ir,jr,kr->ijk", lam, *[y.astype(np.float64) for y in Y])
+        # the model is compared on the whole grid (zeros included), as CP-ALS fits it
+        D = dense(T)
+        fit = 1 - np.linalg.norm(D - M) / np.linalg.norm(D)
+        assert abs(fits[-1] - fit) < 1e-5, (jacobi, fits[-1], fit)
+
+
+def test_cpals_first_sweep_equals_normal_equations(oracle):
+    """One Gauss-Seidel sweep in FP64 equals numpy's solve of the normal
+    equations mode by mode, with L2 column normalisation."""
+    rng = np.random.default_rng(8)
+    T = rand_tensor(rng, (8, 7, 6), 150)
+    R = 4
+    Y0 = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
+    _, Y, lam = oracle.cpals(T, Y0, 1, 0.0, ridge=0.0, fp64=True)
+    D = dense(T)
+    Ys = [y.astype(np.float64) for y in Y0]
+    for n in range(3):
+        V = np.ones((R, R))
+        for m in range(3):
+            if m != n:
+                V *= Ys[m].T @ Ys[m]
+        X = np.linalg.solve(V, mttkrp_dense(D, Ys, n).T).T
+        l = np.linalg.norm(X, axis=0)
+        Ys[n] = X / l
+    for n in range(3):
+        assert frob_rel(Y[n], Ys[n]) < 1e-5
+    np.testing.assert_allclose(lam, l, rtol=1e-6)
+
+
+def test_jacobi_stale_fit_reproduces_reference_print(oracle):
+    z = golden("cpals_synth_cp_r5_jacobi_r5.npz")
+    T = tensor_from_npz(z)
+    Y0 = [z[f"init{n}"] for n in range(3)]
+    fits_stale, _, _ = oracle.cpals(T, Y0, int(z["iters"]), 0.0, jacobi=True,
+                                    stale_jacobi_fit=True)
+    np.testing.assert_allclose(fits_stale[:3], z["fits"][:3], rtol=0, atol=2e-5)
+    fits_true, Y, lam = oracle.cpals(T, Y0, int(z["iters"]), 0.0, jacobi=True, fp64=True)
+    M = np.einsum("r,ir,jr,kr->ijk", lam, *[y.astype(np.float64) for y in Y])
+    D = dense(T)
+    assert abs(fits_true[-1] - (1 - np.linalg.norm(D - M) / np.linalg.norm(D))) < 1e-5
+
+
+def test_remap_invariants(oracle):
+    rng = np.random.default_rng(9)
+    T = rand_tensor(rng, (13, 11, 17, 5), 600)
+    for mode in range(4):
+        for sec in (-1, (mode + 1) % 4):
+            perm, rowptr = oracle.remap(T, mode, sec)
+            assert sorted(perm.tolist()) == list(range(T.nnz))
+            key = T.idx[mode][perm]
+            assert (np.diff(key.astype(np.int64)) >= 0).all()
+            np.testing.assert_array_equal(
+                rowptr, np.concatenate([[0], np.cumsum(np.bincount(T.idx[mode],
+                                                                   minlength=T.dims[mode]))]))
+            if sec < 0:   # stable: input order inside a row
+                for r in range(T.dims[mode]):
+                    seg = perm[rowptr[r]:rowptr[r + 1]]
+                    assert (np.diff(seg.astype(np.int64)) > 0).all()
+            else:
+                comp = key.astype(np.int64) * 1000 + T.idx[sec][perm]
+                assert (np.diff(comp) >= 0).all()
+
+
+def test_forward_and_completion_loss(oracle):
+    rng = np.random.default_rng(10)
+    T = rand_tensor(rng, (6, 5, 4), 60)
+    A = [rng.standard_normal((d, 3)) for d in T.dims]
+    lam = rng.random(3)
+    m = oracle.forward(T, A, lam)
+    M = np.einsum("r,ir,jr,kr->ijk", lam, *A)
+    ix = tuple(c.astype(np.int64) for c in T.idx)
+    np.testing.assert_allclose(m, M[ix], rtol=1e-12)
+    m1 = np.einsum("ir,jr,kr->ijk", *A)[ix]
+    y = T.vals.astype(np.float64)
+    ls = oracle.completion_loss(T, A, 0)
+    np.testing.assert_allclose(ls, 0.5 * np.sum((m1 - y) ** 2), rtol=1e-12)
+    bl = oracle.completion_loss(T, A, 1)
+    np.testing.assert_allclose(bl, np.sum(np.logaddexp(0, m1) - y * m1), rtol=1e-12)
