@@ -35,31 +35,26 @@ def frob_rel(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
-def skewed_tensor(rng, dims, nnz, heavy_frac=0.4, val_kind="normal"):
-    """Distinct coordinates; a `heavy_frac` share of the nonzeros in row 0 of
-    every mode, so rows span lane groups, workgroups and the fix-up pass."""
-    N = len(dims)
+def skewed_tensor(rng, dims, nnz, heavy_frac=0.4):
+    """Distinct coordinates; up to a `heavy_frac` share of the nonzeros in row 0
+    of mode 0 (capped at half of that row's cells), so rows span lane groups,
+    workgroups and the fix-up pass."""
+    dims = tuple(int(d) for d in dims)
     cells = int(np.prod(dims, dtype=np.float64))
-    nh = int(nnz * heavy_frac)
-    lin = set()
-    out = []
-    sub = dims[1:]
-    while len(out) < nh:
-        c = (0,) + tuple(int(rng.integers(0, d)) for d in sub)
-        k = np.ravel_multi_index(c, dims)
-        if k not in lin:
-            lin.add(k)
-            out.append(k)
-    while len(out) < nnz:
-        k = int(rng.integers(0, cells))
-        if k not in lin:
-            lin.add(k)
-            out.append(k)
-    out = np.array(out, dtype=np.int64)
-    rng.shuffle(out)
-    idx = [i.astype(np.uint32) for i in np.unravel_index(out, dims)]
+    sub = int(np.prod(dims[1:], dtype=np.float64))
+    nh = min(int(nnz * heavy_frac), sub // 2)
+    heavy = rng.choice(sub, size=nh, replace=False).astype(np.int64)   # row 0: linear < sub
+    chosen = heavy
+    while chosen.shape[0] < nnz:
+        need = nnz - chosen.shape[0]
+        cand = np.unique(rng.integers(0, cells, size=2 * need + 16, dtype=np.int64))
+        cand = np.setdiff1d(cand, chosen, assume_unique=False)
+        rng.shuffle(cand)
+        chosen = np.concatenate([chosen, cand[:need]])
+    rng.shuffle(chosen)
+    idx = [i.astype(np.uint32) for i in np.unravel_index(chosen, dims)]
     vals = rng.standard_normal(nnz).astype(np.float32)
-    return Tensor(tuple(dims), vals, idx)
+    return Tensor(dims, vals, idx)
 
 
 @pytest.mark.parametrize("name", VERIFY)
