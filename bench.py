@@ -70,6 +70,10 @@ def cpu_baseline(T, wl, threads, iters=3):
                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env,
                            timeout=600)
         wall = time.time() - t0
+    for line in p.stdout.splitlines():
+        if line.startswith(("[io]", "[layout] NCopy", "[flycoo] preprocess", "[cpals] iter",
+                            "[summary]")):
+            log("ref>", line)
     if p.returncode != 0:
         log("reference CPU run failed:", p.stdout[-500:])
         return None

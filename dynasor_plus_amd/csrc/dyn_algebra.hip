@@ -89,9 +89,14 @@ int tensor_norm2(dyn_gpu_ctx* c, double* out) {
 //  with 16x16x4, row (l / 16) of a row quad at column (l % 16).
 // ---------------------------------------------------------------------------
 struct ScaleArgs {
-    float*        Y;          // rows x RP, scaled in place when inv != null
+    float*        Y;          // rows x RP, scaled in place when sq_part != null
     bf16*         Ybf;        // BF16 shadow to refresh (nullable)
-    const double* inv;        // 1 / lambda per column (nullable: Gram only)
+    const double* sq_part;    // [sq_nwg][RP] column sums of x^2 from the solve
+                              // (nullable: Gram only, no scaling)
+    uint32_t      sq_nwg;
+    int           R;
+    double*       lambda_out; // written by workgroup 0 when scaling
+    double*       inv_out;
     const float*  Yhat;       // non-null: accumulate <Y, Yhat> per column
     float*        gpart;      // [nwg][RP][RP]
     double*       ipart;      // [nwg][RP]
@@ -105,24 +110,51 @@ template <int RP>
 __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
     __shared__ float tile[RP * RP];
     __shared__ double ired[kBlock / 64][RP];
+    __shared__ double s_inv[RP];
+    __shared__ double s_red[kBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     constexpr int W = kBlock / 64;
+
+    // lambda_r = ||X[:, r]||_2 from the solve's per-workgroup partials: every
+    // workgroup repeats this small reduction (fixed order: strided subsets,
+    // then the subsets in order) instead of paying a separate launch
+    const bool scale = a.sq_part != nullptr;
+    if (scale) {
+        constexpr int NS = kBlock / RP;
+        const int col = tid % RP, sub = tid / RP;
+        double t = 0.0;
+        for (uint32_t b = sub; b < a.sq_nwg; b += NS) t += a.sq_part[(size_t)b * RP + col];
+        s_red[tid] = t;
+        __syncthreads();
+        if (tid < RP) {
+            double v = 0.0;
+            for (int j = 0; j < NS; ++j) v += s_red[j * RP + tid];
+            const double l = (tid < a.R) ? sqrt(v) : 0.0;
+            const double iv = (l > 1e-20) ? 1.0 / l : 0.0;
+            s_inv[tid] = iv;
+            if (blockIdx.x == 0) {
+                a.lambda_out[tid] = l;
+                a.inv_out[tid] = iv;
+            }
+        }
+        __syncthreads();
+    }
 
     if constexpr (RP == 16) {
         // 16x16x4: one tile; each wave step consumes 4 rows
         v4f acc = {0.f, 0.f, 0.f, 0.f};
         const int col = lane & 15;
         const int rsub = lane >> 4;
-        const double inv = a.inv ? a.inv[col] : 1.0;
+        const double inv = scale ? s_inv[col] : 1.0;
         double ip = 0.0;
         for (uint32_t r0 = (blockIdx.x * W + wave) * 4u; r0 < a.rows; r0 += a.nwg * W * 4u) {
             const uint32_t r = r0 + rsub;
             float y = 0.f;
             if (r < a.rows) {
                 y = a.Y[(size_t)r * RP + col];
-                if (a.inv) {
+                if (scale) {
                     y = (float)((double)y * inv);
                     a.Y[(size_t)r * RP + col] = y;
                 }
@@ -163,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
         const int rsub = lane >> 5;
         double inv[TB];
 #pragma unroll
-        for (int b = 0; b < TB; ++b) inv[b] = a.inv ? a.inv[b * 32 + col] : 1.0;
+        for (int b = 0; b < TB; ++b) inv[b] = scale ? s_inv[b * 32 + col] : 1.0;
         double ip[TB];
 #pragma unroll
         for (int b = 0; b < TB; ++b) ip[b] = 0.0;
@@ -180,13 +212,13 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
                 if (r < a.rows) {
                     const size_t off = (size_t)r * RP + b * 32 + col;
                     float x = a.Y[off];
-                    if (a.inv) x = (float)((double)x * inv[b]);
+                    if (scale) x = (float)((double)x * inv[b]);
                     y[b] = x;
                     // in the block-row split every wave reads the whole row;
                     // wave b alone writes column block b
                     const bool writer = kSplitRows || wave == b;
                     if (writer) {
-                        if (a.inv) a.Y[off] = x;
+                        if (scale) a.Y[off] = x;
                         if (a.Ybf) a.Ybf[off] = to_bf16(x);
                         if (a.Yhat) ip[b] += (double)x * (double)a.Yhat[off];
                     }
@@ -266,14 +298,20 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
     }
 }
 
+// G = sum of the per-workgroup FP32 Gram partials in FP64.  Block of 256:
+// 64 entries x 4 interleaved partial subsets, combined in subset order.
 template <int RP>
-__global__ void k_gram_reduce(const float* __restrict__ gpart, uint32_t nwg,
-                              double* __restrict__ G) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= RP * RP) return;
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(const float* __restrict__ gpart,
+                                                        uint32_t nwg, double* __restrict__ G) {
+    __shared__ double red[4][64];
+    const int le = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + le;
     double s = 0.0;
-    for (uint32_t b = 0; b < nwg; ++b) s += (double)gpart[(size_t)b * RP * RP + e];
-    G[e] = s;
+    if (e < RP * RP)
+        for (uint32_t b = sub; b < nwg; b += 4) s += (double)gpart[(size_t)b * RP * RP + e];
+    red[sub][le] = s;
+    __syncthreads();
+    if (sub == 0 && e < RP * RP) G[e] = ((red[0][le] + red[1][le]) + red[2][le]) + red[3][le];
 }
 
 // ---------------------------------------------------------------------------
@@ -327,22 +365,24 @@ __global__ __launch_bounds__(kBlock) void k_chol(CholArgs a) {
     const double eps = fmax(a.ridge * s_dmax, 0.0);
     if (tid < R) V[tid * RP + tid] += eps;
     __syncthreads();
+    // right-looking factorisation, one row per thread: in step j thread i
+    // forms V[i][j] - sum_{k<j} V[i][k] V[j][k] (thread j: the pivot), then
+    // the below-diagonal entries are scaled by 1 / pivot
     for (int j = 0; j < R; ++j) {
-        if (tid == 0) {
-            double s = V[j * RP + j];
-            for (int k = 0; k < j; ++k) s -= V[j * RP + k] * V[j * RP + k];
-            if (s < 0.0) s = 0.0;
-            const double d = sqrt(s);
+        const int i = tid;
+        double t = 0.0;
+        if (i >= j && i < R) {
+            t = V[i * RP + j];
+            for (int k = 0; k < j; ++k) t -= V[i * RP + k] * V[j * RP + k];
+        }
+        if (i == j) {
+            const double sp = (t < 0.0) ? 0.0 : t;
+            const double d = sqrt(sp);
             V[j * RP + j] = d;
             s_inv = (d > 0.0) ? 1.0 / d : 0.0;
         }
         __syncthreads();
-        const double inv = s_inv;
-        for (int i = j + 1 + tid; i < R; i += kBlock) {
-            double t = V[i * RP + j];
-            for (int k = 0; k < j; ++k) t -= V[i * RP + k] * V[j * RP + k];
-            V[i * RP + j] = t * inv;
-        }
+        if (i > j && i < R) V[i * RP + j] = t * s_inv;
         __syncthreads();
     }
     // column j of M: L m = e_j with zero pivots -> zero components
@@ -447,17 +487,6 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve(const float* __restrict__
     }
 }
 
-__global__ void k_lambda(const double* __restrict__ sq_part, uint32_t nwg, int R, int RP,
-                         double* __restrict__ lambda, double* __restrict__ inv) {
-    const int c = threadIdx.x;
-    if (c >= RP) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nwg; ++b) s += sq_part[(size_t)b * RP + c];
-    const double l = (c < R) ? sqrt(s) : 0.0;
-    lambda[c] = l;
-    inv[c] = (l > 1e-20) ? 1.0 / l : 0.0;
-}
-
 // fit: out[0] = fit, [1] = residual, [2] = <T,M>, [3] = ||M||^2
 struct FitArgs {
     const double* G[kMaxModes];
@@ -476,33 +505,36 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
     __shared__ double red[kBlock];
     __shared__ double s_inner;
     const int tid = threadIdx.x;
-    // <T, M>
-    double part = 0.0;
+    // <T, M> = sum_r lambda_r * colsum_r; column sums over the partial blocks
+    // in strided subsets, subsets combined in order
     if (a.ipart) {
-        for (int c = tid; c < a.R; c += kBlock) {
-            double s = 0.0;
-            for (uint32_t b = 0; b < a.nwg; ++b) s += a.ipart[(size_t)b * a.RP + c];
-            part += a.lambda[c] * s;
-        }
-    }
-    red[tid] = part;
-    __syncthreads();
-    if (tid == 0) {
+        const int NS = kBlock / a.RP;
+        const int col = tid % a.RP, sub = tid / a.RP;
         double t = 0.0;
-        for (int i = 0; i < kBlock; ++i) t += red[i];
-        s_inner = a.ipart ? t : *a.inner_dev;
+        if (sub < NS)
+            for (uint32_t b = sub; b < a.nwg; b += NS) t += a.ipart[(size_t)b * a.RP + col];
+        red[tid] = t;
+        __syncthreads();
+        if (tid == 0) {
+            double inner = 0.0;
+            for (int c = 0; c < a.R; ++c) {
+                double cs = 0.0;
+                for (int j = 0; j < NS; ++j) cs += red[j * a.RP + c];
+                inner += a.lambda[c] * cs;
+            }
+            s_inner = inner;
+        }
+    } else if (tid == 0) {
+        s_inner = *a.inner_dev;
     }
     __syncthreads();
-    // ||M||^2 = sum_r lambda_r sum_c lambda_c prod_n G_n[r, c]
+    // ||M||^2 = sum_{r,c} lambda_r lambda_c prod_n G_n[r, c]
     double rs = 0.0;
-    for (int r = tid; r < a.R; r += kBlock) {
-        double rowsum = 0.0;
-        for (int c = 0; c < a.R; ++c) {
-            double h = 1.0;
-            for (int n = 0; n < a.N; ++n) h *= a.G[n][(size_t)r * a.RP + c];
-            rowsum += a.lambda[c] * h;
-        }
-        rs += a.lambda[r] * rowsum;
+    for (int e = tid; e < a.R * a.R; e += kBlock) {
+        const int r = e / a.R, c = e % a.R;
+        double h = 1.0;
+        for (int n = 0; n < a.N; ++n) h *= a.G[n][(size_t)r * a.RP + c];
+        rs += a.lambda[r] * (a.lambda[c] * h);
     }
     red[tid] = rs;
     __syncthreads();
@@ -540,18 +572,29 @@ int refresh_bf16(dyn_gpu_ctx* c, int mode) {
 //  host wrappers
 // ---------------------------------------------------------------------------
 static uint32_t gram_wgs(uint32_t rows, int RP) {
-    const uint32_t rows_per_step = (RP == 16) ? 16u : (RP <= 64 ? 8u : 2u);
-    uint32_t want = (rows + rows_per_step * 16 - 1) / (rows_per_step * 16);
+    // ~1024 rows per workgroup: enough workgroups to stream large factors,
+    // few enough partial Grams for the reduction
+    const uint32_t per = (RP <= 64) ? 1024u : 512u;
+    uint32_t want = (rows + per - 1) / per;
     return std::max<uint32_t>(1, std::min<uint32_t>(want, kMaxGramWG));
 }
 
+struct SolveParts {
+    const double* part = nullptr;
+    uint32_t      nwg = 0;
+};
+
 template <int RP>
-static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const double* inv, bool inner, bool bf,
+static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner, bool bf,
                          double** ipart_out, uint32_t* nwg_out) {
     ScaleArgs a;
     a.Y = c->Y[mode];
     a.Ybf = bf ? c->Ybf[mode] : nullptr;
-    a.inv = inv;
+    a.sq_part = sq ? sq->part : nullptr;
+    a.sq_nwg = sq ? sq->nwg : 0;
+    a.R = c->R;
+    a.lambda_out = c->lambda;
+    a.inv_out = c->inv_lambda;
     a.Yhat = inner ? c->Yhat[mode] : nullptr;
     a.rows = c->dims[mode];
     a.nwg = gram_wgs(a.rows, RP);
@@ -563,7 +606,7 @@ static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const double* inv, bool inner
     a.gpart = (float*)((char*)sp + (size_t)a.nwg * RP * sizeof(double));
     hipLaunchKernelGGL(k_scale_gram<RP>, dim3(a.nwg), dim3(kBlock), 0, c->stream, a);
     DYN_CHECK(check_launch(c, "scale_gram"));
-    hipLaunchKernelGGL(k_gram_reduce<RP>, dim3((RP * RP + kBlock - 1) / kBlock), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_gram_reduce<RP>, dim3((RP * RP + 63) / 64), dim3(kBlock), 0,
                        c->stream, a.gpart, a.nwg, c->G[mode]);
     DYN_CHECK(check_launch(c, "gram_reduce"));
     if (ipart_out) *ipart_out = a.ipart;
@@ -571,13 +614,13 @@ static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const double* inv, bool inner
     return DYN_OK;
 }
 
-static int scale_gram(dyn_gpu_ctx* c, int mode, const double* inv, bool inner, bool bf,
+static int scale_gram(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner, bool bf,
                       double** ipart_out, uint32_t* nwg_out) {
     switch (c->Rp) {
-        case 16: return scale_gram_rp<16>(c, mode, inv, inner, bf, ipart_out, nwg_out);
-        case 32: return scale_gram_rp<32>(c, mode, inv, inner, bf, ipart_out, nwg_out);
-        case 64: return scale_gram_rp<64>(c, mode, inv, inner, bf, ipart_out, nwg_out);
-        case 128: return scale_gram_rp<128>(c, mode, inv, inner, bf, ipart_out, nwg_out);
+        case 16: return scale_gram_rp<16>(c, mode, sq, inner, bf, ipart_out, nwg_out);
+        case 32: return scale_gram_rp<32>(c, mode, sq, inner, bf, ipart_out, nwg_out);
+        case 64: return scale_gram_rp<64>(c, mode, sq, inner, bf, ipart_out, nwg_out);
+        case 128: return scale_gram_rp<128>(c, mode, sq, inner, bf, ipart_out, nwg_out);
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
     }
 }
@@ -586,7 +629,8 @@ int gram_mode(dyn_gpu_ctx* c, int mode) {
     return scale_gram(c, mode, nullptr, false, false, nullptr, nullptr);
 }
 
-static int chol(dyn_gpu_ctx* c, int n, double ridge) {
+// V_n^{-1} into c->L on `stream`
+static int chol(dyn_gpu_ctx* c, int n, double ridge, hipStream_t stream) {
     CholArgs a;
     for (int m = 0; m < c->N; ++m) a.G[m] = c->G[m];
     a.N = c->N;
@@ -595,16 +639,16 @@ static int chol(dyn_gpu_ctx* c, int n, double ridge) {
     a.ridge = ridge;
     a.W = c->L;
     switch (c->Rp) {
-        case 16: hipLaunchKernelGGL(k_chol<16>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
-        case 32: hipLaunchKernelGGL(k_chol<32>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
-        case 64: hipLaunchKernelGGL(k_chol<64>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
-        case 128: hipLaunchKernelGGL(k_chol<128>, dim3(1), dim3(kBlock), 0, c->stream, a); break;
+        case 16: hipLaunchKernelGGL(k_chol<16>, dim3(1), dim3(kBlock), 0, stream, a); break;
+        case 32: hipLaunchKernelGGL(k_chol<32>, dim3(1), dim3(kBlock), 0, stream, a); break;
+        case 64: hipLaunchKernelGGL(k_chol<64>, dim3(1), dim3(kBlock), 0, stream, a); break;
+        case 128: hipLaunchKernelGGL(k_chol<128>, dim3(1), dim3(kBlock), 0, stream, a); break;
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
     }
     return check_launch(c, "chol");
 }
 
-static int solve_and_lambda(dyn_gpu_ctx* c, int n) {
+static int solve(dyn_gpu_ctx* c, int n, SolveParts* out) {
     const uint32_t rows = c->dims[n];
     const uint32_t nwg = std::max<uint32_t>(
         1, std::min<uint32_t>((rows + kSolveBlock - 1) / kSolveBlock, kMaxSolveWG));
@@ -618,10 +662,9 @@ static int solve_and_lambda(dyn_gpu_ctx* c, int n) {
         case 128: hipLaunchKernelGGL(k_solve<128>, dim3(nwg), dim3(kSolveBlock), 0, c->stream, c->Yhat[n], c->Y[n], c->L, rows, part); break;
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
     }
-    DYN_CHECK(check_launch(c, "solve"));
-    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(kMaxRp), 0, c->stream, part, nwg, c->R, c->Rp,
-                       c->lambda, c->inv_lambda);
-    return check_launch(c, "lambda");
+    out->part = part;
+    out->nwg = nwg;
+    return check_launch(c, "solve");
 }
 
 static int fit_eval(dyn_gpu_ctx* c, const double* ipart, uint32_t nwg, const double* inner_dev,
@@ -702,9 +745,16 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     }
 
     hipEvent_t ev[2 * kMaxModes];
-    for (int i = 0; i < 2 * kMaxModes; ++i) DYN_HIP(c, hipEventCreate(&ev[i]));
+    for (int i = 0; i < 2 * kMaxModes; ++i) DYN_HIP(c, hipEventCreateWithFlags(&ev[i], 0));
+    // the Hadamard / Cholesky / inverse of mode n depends only on the Grams,
+    // so it runs on the auxiliary stream while mode n's spMTTKRP runs
+    hipEvent_t ev_g, ev_l;
+    DYN_HIP(c, hipEventCreateWithFlags(&ev_g, hipEventDisableTiming));
+    DYN_HIP(c, hipEventCreateWithFlags(&ev_l, hipEventDisableTiming));
     auto destroy_events = [&]() {
         for (int i = 0; i < 2 * kMaxModes; ++i) (void)hipEventDestroy(ev[i]);
+        (void)hipEventDestroy(ev_g);
+        (void)hipEventDestroy(ev_l);
     };
 
     int rc = DYN_OK;
@@ -726,6 +776,17 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         double* ipart = nullptr;
         uint32_t inwg = 0;
         for (int n = 0; n < N && rc == DYN_OK; ++n) {
+            // Grams current -> V_n^{-1} on the auxiliary stream
+            if (hipEventRecord(ev_g, c->stream) != hipSuccess ||
+                hipStreamWaitEvent(c->aux_stream, ev_g, 0) != hipSuccess) {
+                rc = set_err(c, DYN_ERR_HIP, "event record/wait failed");
+                break;
+            }
+            if ((rc = chol(c, n, o->ridge, c->aux_stream)) != DYN_OK) break;
+            if (hipEventRecord(ev_l, c->aux_stream) != hipSuccess) {
+                rc = set_err(c, DYN_ERR_HIP, "event record failed");
+                break;
+            }
             if (!jacobi) {
                 (void)hipEventRecord(ev[2 * n], c->stream);
                 rc = mttkrp_mode(c, n, o->precision, Yin, nullptr, c->Yhat[n], c->Rp);
@@ -733,11 +794,14 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                 nev = n + 1;
                 if (rc != DYN_OK) break;
             }
-            if ((rc = chol(c, n, o->ridge)) != DYN_OK) break;
-            if ((rc = solve_and_lambda(c, n)) != DYN_OK) break;
+            if (hipStreamWaitEvent(c->stream, ev_l, 0) != hipSuccess) {
+                rc = set_err(c, DYN_ERR_HIP, "stream wait failed");
+                break;
+            }
+            SolveParts sq;
+            if ((rc = solve(c, n, &sq)) != DYN_OK) break;
             const bool last = (n == N - 1) && !jacobi;
-            rc = scale_gram(c, n, c->inv_lambda, last, bf, last ? &ipart : nullptr,
-                            last ? &inwg : nullptr);
+            rc = scale_gram(c, n, &sq, last, bf, last ? &ipart : nullptr, last ? &inwg : nullptr);
         }
         if (rc != DYN_OK) break;
         if (jacobi) {

@@ -196,12 +196,18 @@ extern "C" int dyn_gpu_create(int device, dyn_gpu_ctx** out) {
         return DYN_ERR_HIP;
     }
     c->stream = c->own_stream;
+    if (hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->own_stream);
+        delete c;
+        return DYN_ERR_HIP;
+    }
     if (const char* cap = std::getenv("DYN_GPU_MEM_CAP_MB")) {
         long long mb = std::atoll(cap);
         if (mb > 0) c->mem_cap = (uint64_t)mb << 20;
     }
     if (hipHostMalloc((void**)&c->fitbuf_host, 16 * sizeof(double), hipHostMallocDefault) !=
         hipSuccess) {
+        (void)hipStreamDestroy(c->aux_stream);
         (void)hipStreamDestroy(c->own_stream);
         delete c;
         return DYN_ERR_HIP;
@@ -228,6 +234,8 @@ extern "C" int dyn_gpu_destroy(dyn_gpu_ctx* c) {
     }
     for (auto& kv : c->allocs) (void)hipFree(kv.first);
     (void)hipHostFree(c->fitbuf_host);
+    (void)hipStreamSynchronize(c->aux_stream);
+    (void)hipStreamDestroy(c->aux_stream);
     (void)hipStreamDestroy(c->own_stream);
     delete c;
     return DYN_OK;

@@ -53,6 +53,7 @@ struct dyn_gpu_ctx {
     int          device = 0;
     hipStream_t  own_stream = nullptr;
     hipStream_t  stream = nullptr;
+    hipStream_t  aux_stream = nullptr;   // CP-ALS: R x R work overlapped with spMTTKRP
     std::string  err;
 
     // memory accounting

@@ -9,17 +9,19 @@
 // fused multiply-add with Y[w_last] into the row accumulator.
 //
 // Work decomposition (gfx950, wave64):
-//   * a lane group of G = Rp/4 lanes owns one nonzero at a time; lane q
-//     holds columns [4q, 4q+4) of the row as a float4 (BF16 operands: 4
-//     bf16 = 8 bytes per lane, widened to FP32),
-//   * each group walks kSeg consecutive nonzeros of the sorted copy; the
-//     value and index columns are loaded G at a time (one element per lane,
-//     contiguous) and broadcast inside the group with lane shuffles,
+//   * a quad (4 lanes) owns one nonzero at a time; lane q holds Rp/4 columns
+//     of the row, loaded 16 bytes at a time (4 FP32 or 8 BF16 columns per
+//     load, widened to FP32), so one quad reads a whole factor row per
+//     operand and the per-nonzero bookkeeping is paid by 4 lanes only,
+//   * each quad walks kSeg consecutive nonzeros of the sorted copy; the value
+//     and index columns are loaded 4 at a time (one element per lane,
+//     contiguous) and broadcast inside the quad with DPP quad permutes; the
+//     gathers of a batch are issued before its arithmetic,
 //   * the output row is accumulated in registers while the row pointer says
-//     the row continues; a row that starts and ends inside the group's range
+//     the row continues; a row that starts and ends inside the quad's range
 //     is stored directly,
-//   * rows cut by a group boundary leave a partial in LDS; one workgroup-level
-//     pass sums partials of the same row in group order; rows cut by a
+//   * rows cut by a quad boundary leave a partial in LDS; one workgroup-level
+//     pass sums partials of the same row in quad order; rows cut by a
 //     workgroup boundary leave a partial in a global table that a fix-up
 //     kernel sums in workgroup order.  No atomics: results are bitwise
 //     reproducible run to run.
@@ -30,6 +32,8 @@
 namespace dyn {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kQuads = kBlock / 4;            // quads per workgroup
+constexpr uint32_t kChunk = (uint32_t)kQuads * kSeg;   // nonzeros per workgroup
 
 struct SegArgs {
     const float*    vals;         // copy values, or the override vector
@@ -49,29 +53,79 @@ struct SegArgs {
     uint32_t        nwg;
 };
 
-__device__ __forceinline__ float4 ld4(const float* p) {
-    return *reinterpret_cast<const float4*>(p);
+// ---------------------------------------------------------------------------
+//  lane <-> column mapping.  A row is read with 16-byte loads; load b of lane
+//  q covers columns [b*4*LW + q*LW, +LW) with LW = 4 (FP32) or 8 (BF16, or 4
+//  when the row is only 16 wide).  The same mapping is used for the
+//  accumulators and the row stores.
+// ---------------------------------------------------------------------------
+template <int RP, typename FT>
+struct Cols {
+    static constexpr int LW = (sizeof(FT) == 2 && RP >= 32) ? 8 : 4;
+    static constexpr int NB = RP / (4 * LW);
+    static constexpr int CPL = RP / 4;         // columns per lane
+    static __device__ __forceinline__ int col(int b, int q, int t) { return b * 4 * LW + q * LW + t; }
+};
+
+__device__ __forceinline__ void widen(uint32_t u, float& lo, float& hi) {
+    lo = __uint_as_float(u << 16);
+    hi = __uint_as_float(u & 0xFFFF0000u);
 }
-__device__ __forceinline__ float4 ld4(const bf16* p) {
-    const uint2 u = *reinterpret_cast<const uint2*>(p);
-    float4 r;
-    r.x = __uint_as_float(u.x << 16);
-    r.y = __uint_as_float(u.x & 0xFFFF0000u);
-    r.z = __uint_as_float(u.y << 16);
-    r.w = __uint_as_float(u.y & 0xFFFF0000u);
-    return r;
+
+template <int RP, typename FT>
+__device__ __forceinline__ void load_row(const FT* __restrict__ row, int q, float* v) {
+    using C = Cols<RP, FT>;
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b) {
+        if constexpr (sizeof(FT) == 4) {
+            const float4 x = *reinterpret_cast<const float4*>(
+                reinterpret_cast<const float*>(row) + C::col(b, q, 0));
+            v[b * 4 + 0] = x.x;
+            v[b * 4 + 1] = x.y;
+            v[b * 4 + 2] = x.z;
+            v[b * 4 + 3] = x.w;
+        } else if constexpr (C::LW == 8) {
+            const uint4 x = *reinterpret_cast<const uint4*>(
+                reinterpret_cast<const uint16_t*>(row) + C::col(b, q, 0));
+            widen(x.x, v[b * 8 + 0], v[b * 8 + 1]);
+            widen(x.y, v[b * 8 + 2], v[b * 8 + 3]);
+            widen(x.z, v[b * 8 + 4], v[b * 8 + 5]);
+            widen(x.w, v[b * 8 + 6], v[b * 8 + 7]);
+        } else {
+            const uint2 x = *reinterpret_cast<const uint2*>(
+                reinterpret_cast<const uint16_t*>(row) + C::col(b, q, 0));
+            widen(x.x, v[b * 4 + 0], v[b * 4 + 1]);
+            widen(x.y, v[b * 4 + 2], v[b * 4 + 3]);
+        }
+    }
 }
-__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
-__device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
-__device__ __forceinline__ float4 add4(float4 a, float4 b) {
-    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+
+template <int RP, typename FT>
+__device__ __forceinline__ void store_row(float* __restrict__ row, int q, const float* v) {
+    using C = Cols<RP, FT>;
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b)
+#pragma unroll
+        for (int h = 0; h < C::LW / 4; ++h) {
+            const int t = b * C::LW + h * 4;
+            *reinterpret_cast<float4*>(row + C::col(b, q, h * 4)) =
+                make_float4(v[t], v[t + 1], v[t + 2], v[t + 3]);
+        }
 }
-__device__ __forceinline__ float4 mul4(float4 a, float4 b) {
-    return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
+
+// quad broadcast of lane J (DPP quad_perm [J,J,J,J])
+template <int J>
+__device__ __forceinline__ uint32_t qb(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, J * 0x55, 0xF, 0xF, false);
 }
-__device__ __forceinline__ float4 fma4(float4 a, float4 b, float4 c) {
-    return make_float4(__fmaf_rn(a.x, b.x, c.x), __fmaf_rn(a.y, b.y, c.y),
-                       __fmaf_rn(a.z, b.z, c.z), __fmaf_rn(a.w, b.w, c.w));
+template <int J>
+__device__ __forceinline__ float qbf(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J * 0x55, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_sum(float x) {
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+    x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));
+    return x;
 }
 
 // Hardware dispatches workgroup i to XCD i % 8; hand each XCD a contiguous
@@ -98,24 +152,38 @@ __device__ __forceinline__ double lossval(int kind, float m, float y) {
     return 0.5 * d * d;
 }
 
+// nonzeros of a 4-batch whose gathers are issued together
+template <int RP, int NO>
+constexpr int batch_hoist() {
+    constexpr int regs = NO * (RP / 4);
+    return regs >= 64 ? 1 : (regs >= 32 ? 2 : 4);
+}
+
+template <int J, int NO>
+__device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, float& v, uint32_t* iw) {
+    v = qbf<J>(v_l);
+#pragma unroll
+    for (int w = 0; w < NO; ++w) iw[w] = qb<J>(i_l[w]);
+}
+
 // KIND 0: spMTTKRP / adjoint.  KIND 1: fused completion loss + gradient.
 template <int RP, int NO, typename FT, int KIND>
 __global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
-    constexpr int G = RP / 4;
-    constexpr int NG = kBlock / G;
-    constexpr uint32_t CH = (uint32_t)NG * kSeg;
-    __shared__ float4 pacc[2 * NG][G];
-    __shared__ uint32_t prow[2 * NG];
+    using C = Cols<RP, FT>;
+    constexpr int CPL = C::CPL;
+    constexpr int HB = batch_hoist<RP, NO>();
+    __shared__ float pacc[2 * kQuads][RP];
+    __shared__ uint32_t prow[2 * kQuads];
     __shared__ double lred[kBlock / 64];
 
     const int tid = threadIdx.x;
-    const int g = tid / G;
-    const int q = tid % G;
+    const int g = tid >> 2;
+    const int q = tid & 3;
     const uint32_t wg = xcd_chunk(blockIdx.x, a.nwg);
-    const uint32_t wg_s = wg * CH;
-    const uint32_t wg_e = min(wg_s + CH, a.nnz);
+    const uint32_t wg_s = wg * kChunk;
+    const uint32_t wg_e = min(wg_s + kChunk, a.nnz);
 
-    if (tid < 2 * NG) prow[tid] = kNone;
+    if (tid < 2 * kQuads) prow[tid] = kNone;
     if (tid < 2) a.brow[2 * wg + tid] = kNone;
     __syncthreads();
 
@@ -127,86 +195,119 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
         uint32_t row = a.seg[s / kSeg];
         uint32_t rbeg = a.rowptr[row];
         uint32_t rend = a.rowptr[row + 1];
-        float4 acc = f4(0.f);
-        float4 arow = f4(0.f);
-        if (KIND == 1) arow = ld4(a.An + (size_t)row * RP + q * 4);
+        float acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
+        float arow[KIND == 1 ? CPL : 1];
+        if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
 
         auto flush = [&](uint32_t r, uint32_t rb, uint32_t re) {
             const bool os = rb < s, oe = re > e;
             if (!os && !oe) {
-                st4(a.out + (size_t)r * RP + q * 4, acc);
+                store_row<RP, FT>(a.out + (size_t)r * RP, q, acc);
             } else {
                 const int slot = 2 * g + (os ? 0 : 1);
-                pacc[slot][q] = acc;
+                store_row<RP, FT>(&pacc[slot][0], q, acc);
                 if (q == 0) prow[slot] = r;
             }
         };
 
-        for (uint32_t k0 = s; k0 < e; k0 += G) {
-            const uint32_t kk = k0 + q;
-            float v_l = 0.f;
-            uint32_t i_l[NO];
+        auto step = [&](uint32_t k, float v, const float (*y)[CPL]) {
+            if (k >= rend) {
+                flush(row, rbeg, rend);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
+                do {
+                    ++row;
+                    rbeg = rend;
+                    rend = a.rowptr[row + 1];
+                } while (k >= rend);
+                if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
+            }
+            if constexpr (KIND == 0) {
+                if constexpr (NO == 1) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(v, y[0][c], acc[c]);
+                } else {
+                    float p[CPL];
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) p[c] = v * y[0][c];
+#pragma unroll
+                    for (int w = 1; w < NO - 1; ++w)
+#pragma unroll
+                        for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(p[c], y[NO - 1][c], acc[c]);
+                }
+            } else {
+                float p[CPL];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) p[c] = y[0][c];
+#pragma unroll
+                for (int w = 1; w < NO; ++w)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
+                float part = 0.f;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) part = __fmaf_rn(p[c], arow[c], part);
+                const float m = quad_sum(part);
+                const float gr = dloss(a.loss, m, v);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(gr, p[c], acc[c]);
+                if (a.loss_part && q == 0) lsum += lossval(a.loss, m, v);
+            }
+        };
+
+        // value / index columns of the next 4-batch are loaded while the
+        // current batch's gathers are in flight
+        auto fetch = [&](uint32_t kb, float& v_l, uint32_t* i_l) {
+            const uint32_t kk = kb + q;
             if (kk < e) {
                 v_l = a.vperm ? a.vals[a.vperm[kk]] : a.vals[kk];
 #pragma unroll
                 for (int w = 0; w < NO; ++w) i_l[w] = a.oidx[w][kk];
             } else {
+                v_l = 0.f;
 #pragma unroll
-                for (int w = 0; w < NO; ++w) i_l[w] = 0;
+                for (int w = 0; w < NO; ++w) i_l[w] = 0u;
             }
-            const uint32_t cnt = min((uint32_t)G, e - k0);
-#pragma unroll 4
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t k = k0 + j;
-                if (k >= rend) {
-                    flush(row, rbeg, rend);
-                    acc = f4(0.f);
-                    do {
-                        ++row;
-                        rbeg = rend;
-                        rend = a.rowptr[row + 1];
-                    } while (k >= rend);
-                    if (KIND == 1) arow = ld4(a.An + (size_t)row * RP + q * 4);
-                }
-                const float v = __shfl(v_l, (int)j, G);
-                uint32_t iw[NO];
+        };
+        float v_n;
+        uint32_t i_n[NO];
+        fetch(s, v_n, i_n);
+        for (uint32_t k0 = s; k0 < e; k0 += 4) {
+            const float v_l = v_n;
+            uint32_t i_l[NO];
 #pragma unroll
-                for (int w = 0; w < NO; ++w) iw[w] = __shfl(i_l[w], (int)j, G);
-                const FT* y0p = reinterpret_cast<const FT*>(a.Yo[0]) + (size_t)iw[0] * RP + q * 4;
-                if (KIND == 0) {
-                    if (NO == 1) {
-                        acc = fma4(f4(v), ld4(y0p), acc);
-                    } else {
-                        float4 p = mul4(f4(v), ld4(y0p));
+            for (int w = 0; w < NO; ++w) i_l[w] = i_n[w];
+            if (k0 + 4 < e) fetch(k0 + 4, v_n, i_n);
+            const uint32_t cnt = min(4u, e - k0);
+            float vb[4];
+            uint32_t ib[4][NO];
+            bcast_batch<0, NO>(v_l, i_l, vb[0], ib[0]);
+            bcast_batch<1, NO>(v_l, i_l, vb[1], ib[1]);
+            bcast_batch<2, NO>(v_l, i_l, vb[2], ib[2]);
+            bcast_batch<3, NO>(v_l, i_l, vb[3], ib[3]);
 #pragma unroll
-                        for (int w = 1; w < NO - 1; ++w)
-                            p = mul4(p, ld4(reinterpret_cast<const FT*>(a.Yo[w]) +
-                                            (size_t)iw[w] * RP + q * 4));
-                        acc = fma4(p, ld4(reinterpret_cast<const FT*>(a.Yo[NO - 1]) +
-                                          (size_t)iw[NO - 1] * RP + q * 4),
-                                   acc);
-                    }
-                } else {
-                    float4 p = ld4(y0p);
+            for (int j0 = 0; j0 < 4; j0 += HB) {
+                float y[HB][NO][CPL];
 #pragma unroll
-                    for (int w = 1; w < NO; ++w)
-                        p = mul4(p, ld4(reinterpret_cast<const FT*>(a.Yo[w]) +
-                                        (size_t)iw[w] * RP + q * 4));
-                    const float4 t = mul4(p, arow);
-                    float m = (t.x + t.y) + (t.z + t.w);
+                for (int h = 0; h < HB; ++h)
 #pragma unroll
-                    for (int o = 1; o < G; o <<= 1) m += __shfl_xor(m, o, G);
-                    const float gr = dloss(a.loss, m, v);
-                    acc = fma4(f4(gr), p, acc);
-                    if (a.loss_part && q == 0) lsum += lossval(a.loss, m, v);
-                }
+                    for (int w = 0; w < NO; ++w)
+                        load_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]) +
+                                             (size_t)ib[j0 + h][w] * RP,
+                                         q, y[h][w]);
+#pragma unroll
+                for (int h = 0; h < HB; ++h)
+                    if ((uint32_t)(j0 + h) < cnt) step(k0 + j0 + h, vb[j0 + h], y[h]);
             }
         }
         flush(row, rbeg, rend);
     }
     __syncthreads();
 
-    // workgroup-level combine of the partial rows, in group order
+    // workgroup-level combine of the partial rows, in quad order
     for (int half = 0; half < 2; ++half) {
         const int slot = 2 * g + half;
         const uint32_t r = prow[slot];
@@ -214,26 +315,32 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
         int p = slot - 1;
         while (p >= 0 && prow[p] == kNone) --p;
         if (p >= 0 && prow[p] == r) continue;   // not the first partial of its row
-        float4 sum = pacc[slot][q];
-        for (int t = slot + 1; t < 2 * NG; ++t) {
-            const uint32_t rt = prow[t];
+        float sum[CPL];
+#pragma unroll
+        for (int b = 0; b < C::NB; ++b)
+#pragma unroll
+            for (int t = 0; t < C::LW; ++t) sum[b * C::LW + t] = pacc[slot][C::col(b, q, t)];
+        for (int t2 = slot + 1; t2 < 2 * kQuads; ++t2) {
+            const uint32_t rt = prow[t2];
             if (rt == kNone) continue;
             if (rt != r) break;
-            sum = add4(sum, pacc[t][q]);
+#pragma unroll
+            for (int b = 0; b < C::NB; ++b)
+#pragma unroll
+                for (int t = 0; t < C::LW; ++t) sum[b * C::LW + t] += pacc[t2][C::col(b, q, t)];
         }
         const bool os = a.rowptr[r] < wg_s;
         const bool oe = a.rowptr[r + 1] > wg_e;
         if (!os && !oe) {
-            st4(a.out + (size_t)r * RP + q * 4, sum);
+            store_row<RP, FT>(a.out + (size_t)r * RP, q, sum);
         } else {
             const int bs = os ? 0 : 1;
-            st4(a.bpart + ((size_t)wg * 2 + bs) * RP + q * 4, sum);
+            store_row<RP, FT>(a.bpart + ((size_t)wg * 2 + bs) * RP, q, sum);
             if (q == 0) a.brow[2 * wg + bs] = r;
         }
     }
 
     if (KIND == 1 && a.loss_part) {
-        // deterministic block sum of the per-group loss (lane q == 0 holds it)
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
         if ((tid & 63) == 0) lred[tid >> 6] = lsum;
@@ -248,33 +355,82 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
 
 // Rows cut by workgroup boundaries: a partial in slot 1 of workgroup b opens
 // the row; slot 0 of workgroups b+1, b+2, ... continue it while the row id
-// matches.  One workgroup per opening partial sums the chain in a fixed
-// order (strided by lane group, then an ordered LDS reduction).
+// matches.  Each quad of a fix-up block owns one workgroup slot; chains of up
+// to kShortChain partials are summed by their quad in order, longer ones
+// (heavy rows) by the whole block: quads stride the chain, then an ordered
+// LDS reduction.  Deterministic.
+constexpr int kShortChain = 8;
+
 template <int RP>
 __global__ __launch_bounds__(kBlock) void k_fixup(const float* __restrict__ bpart,
                                                   const uint32_t* __restrict__ brow,
                                                   uint32_t nwg, float* __restrict__ out) {
-    constexpr int G = RP / 4;
-    constexpr int NG = kBlock / G;
-    __shared__ float4 red[NG][G];
-    const uint32_t b = blockIdx.x;
-    const uint32_t r = brow[2 * b + 1];
-    if (r == kNone) return;
+    using C = Cols<RP, float>;
+    constexpr int CPL = C::CPL;
+    __shared__ float red[kQuads][RP];
+    __shared__ uint32_t longb[kQuads];
+    __shared__ uint32_t nlong;
     const int tid = threadIdx.x;
-    const int g = tid / G;
-    const int q = tid % G;
-    float4 sum = f4(0.f);
-    if (g == 0) sum = ld4(bpart + ((size_t)b * 2 + 1) * RP + q * 4);
-    for (uint32_t c = b + 1 + g; c < nwg; c += NG) {
-        if (brow[2 * c] != r) break;
-        sum = add4(sum, ld4(bpart + ((size_t)c * 2) * RP + q * 4));
-    }
-    red[g][q] = sum;
+    const int g = tid >> 2;
+    const int q = tid & 3;
+    if (tid == 0) nlong = 0;
     __syncthreads();
-    if (g == 0) {
-        float4 t = red[0][q];
-        for (int j = 1; j < NG; ++j) t = add4(t, red[j][q]);
-        st4(out + (size_t)r * RP + q * 4, t);
+    const uint32_t b = blockIdx.x * kQuads + g;
+    if (b < nwg) {
+        const uint32_t r = brow[2 * b + 1];
+        if (r != kNone) {
+            float sum[CPL];
+            load_row<RP, float>(bpart + ((size_t)b * 2 + 1) * RP, q, sum);
+            uint32_t c = b + 1;
+            int len = 0;
+            for (; c < nwg && len < kShortChain; ++c, ++len) {
+                if (brow[2 * c] != r) break;
+                float t[CPL];
+                load_row<RP, float>(bpart + ((size_t)c * 2) * RP, q, t);
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) sum[i] += t[i];
+            }
+            const bool more = (c < nwg) && (len == kShortChain) && (brow[2 * c] == r);
+            if (!more) {
+                store_row<RP, float>(out + (size_t)r * RP, q, sum);
+            } else if (q == 0) {
+                const uint32_t slot = atomicAdd(&nlong, 1u);
+                longb[slot] = b;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t nl = nlong;
+    // process long chains in ascending workgroup order (slot order from the
+    // atomic is arbitrary, the arithmetic below does not depend on it)
+    for (uint32_t i = 0; i < nl; ++i) {
+        const uint32_t lb = longb[i];
+        const uint32_t r = brow[2 * lb + 1];
+        float sum[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) sum[j] = 0.f;
+        if (g == 0) load_row<RP, float>(bpart + ((size_t)lb * 2 + 1) * RP, q, sum);
+        for (uint32_t c = lb + 1 + g; c < nwg; c += kQuads) {
+            if (brow[2 * c] != r) break;
+            float t[CPL];
+            load_row<RP, float>(bpart + ((size_t)c * 2) * RP, q, t);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) sum[j] += t[j];
+        }
+        store_row<RP, float>(&red[g][0], q, sum);
+        __syncthreads();
+        if (g == 0) {
+            float t[CPL];
+            load_row<RP, float>(&red[0][0], q, t);
+            for (int j = 1; j < kQuads; ++j) {
+                float u[CPL];
+                load_row<RP, float>(&red[j][0], q, u);
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) t[k] += u[k];
+            }
+            store_row<RP, float>(out + (size_t)r * RP, q, t);
+        }
+        __syncthreads();
     }
 }
 
@@ -314,11 +470,12 @@ static int dispatch_rp(dyn_gpu_ctx* c, int Rp, int NO, const SegArgs& a) {
 
 static int launch_fixup(dyn_gpu_ctx* c, int Rp, const float* bpart, const uint32_t* brow,
                         uint32_t nwg, float* out) {
+    const unsigned nb = (nwg + kQuads - 1) / kQuads;
     switch (Rp) {
-        case 16: hipLaunchKernelGGL(k_fixup<16>, dim3(nwg), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 32: hipLaunchKernelGGL(k_fixup<32>, dim3(nwg), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 64: hipLaunchKernelGGL(k_fixup<64>, dim3(nwg), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 128: hipLaunchKernelGGL(k_fixup<128>, dim3(nwg), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
+        case 16: hipLaunchKernelGGL(k_fixup<16>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
+        case 32: hipLaunchKernelGGL(k_fixup<32>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
+        case 64: hipLaunchKernelGGL(k_fixup<64>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
+        case 128: hipLaunchKernelGGL(k_fixup<128>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", Rp);
     }
     return check_launch(c, "fixup");
@@ -337,13 +494,11 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
         ++j;
     }
     *NO = j;
-    const uint32_t G = (uint32_t)Rp / 4;
-    const uint32_t CH = (kBlock / G) * kSeg;
     a->vals = cp.vals;
     a->rowptr = cp.rowptr;
     a->seg = cp.seg;
     a->nnz = (uint32_t)c->nnz;
-    a->nwg = (uint32_t)((c->nnz + CH - 1) / CH);
+    a->nwg = (uint32_t)((c->nnz + kChunk - 1) / kChunk);
     void* sp = nullptr;
     const size_t nwg = std::max<uint32_t>(a->nwg, 1);
     DYN_CHECK(scratch2(c, nwg * 2 * (size_t)Rp * sizeof(float) + nwg * 2 * sizeof(uint32_t) +
@@ -391,42 +546,59 @@ struct FwdArgs {
     uint32_t        nnz;
 };
 
+template <int J, int N>
+__device__ __forceinline__ void bcast_idx(const uint32_t* i_l, uint32_t* iw) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) iw[n] = qb<J>(i_l[n]);
+}
+
 template <int RP, int N>
 __global__ __launch_bounds__(kBlock) void k_forward(FwdArgs a) {
-    constexpr int G = RP / 4;
-    constexpr int NG = kBlock / G;
+    using C = Cols<RP, float>;
+    constexpr int CPL = C::CPL;
     __shared__ double red[kBlock / 64];
     const int tid = threadIdx.x;
-    const int g = tid / G;
-    const int q = tid % G;
-    float4 lam = f4(1.f);
-    if (a.lambda) {
-        lam = make_float4((float)a.lambda[q * 4 + 0], (float)a.lambda[q * 4 + 1],
-                          (float)a.lambda[q * 4 + 2], (float)a.lambda[q * 4 + 3]);
-    }
+    const int g = tid >> 2;
+    const int q = tid & 3;
+    float lam[CPL];
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b)
+#pragma unroll
+        for (int t = 0; t < C::LW; ++t)
+            lam[b * C::LW + t] = a.lambda ? (float)a.lambda[C::col(b, q, t)] : 1.f;
     double dot = 0.0;
-    const uint64_t per_block = (uint64_t)NG * G;   // nonzeros per block iteration
-    for (uint64_t k0 = (uint64_t)blockIdx.x * per_block + (uint64_t)g * G; k0 < a.nnz;
+    const uint64_t per_block = (uint64_t)kQuads * 4;
+    for (uint64_t k0 = (uint64_t)blockIdx.x * per_block + (uint64_t)g * 4; k0 < a.nnz;
          k0 += (uint64_t)gridDim.x * per_block) {
         const uint64_t kk = k0 + q;
         uint32_t i_l[N];
 #pragma unroll
         for (int n = 0; n < N; ++n) i_l[n] = (kk < a.nnz) ? a.idx[n][kk] : 0u;
-        float v_l = (a.vals && kk < a.nnz) ? a.vals[kk] : 0.f;
-        const uint32_t cnt = (uint32_t)min((uint64_t)G, a.nnz - k0);
-        float mine = 0.f;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            float4 p = lam;
+        const float v_l = (a.vals && kk < a.nnz) ? a.vals[kk] : 0.f;
+        uint32_t ib[4][N];
+        bcast_idx<0, N>(i_l, ib[0]);
+        bcast_idx<1, N>(i_l, ib[1]);
+        bcast_idx<2, N>(i_l, ib[2]);
+        bcast_idx<3, N>(i_l, ib[3]);
+        float mj[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float p[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) p[c] = lam[c];
 #pragma unroll
             for (int n = 0; n < N; ++n) {
-                const uint32_t ii = __shfl(i_l[n], (int)j, G);
-                p = mul4(p, ld4(a.A[n] + (size_t)ii * RP + q * 4));
-            }
-            float m = (p.x + p.y) + (p.z + p.w);
+                float y[CPL];
+                load_row<RP, float>(a.A[n] + (size_t)ib[j][n] * RP, q, y);
 #pragma unroll
-            for (int o = 1; o < G; o <<= 1) m += __shfl_xor(m, o, G);
-            if ((uint32_t)q == j) mine = m;
+                for (int c = 0; c < CPL; ++c) p[c] *= y[c];
+            }
+            float part = 0.f;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) part += p[c];
+            mj[j] = quad_sum(part);
         }
+        const float mine = (q == 0) ? mj[0] : (q == 1) ? mj[1] : (q == 2) ? mj[2] : mj[3];
         if (kk < a.nnz) {
             if (a.m) a.m[kk] = mine;
             if (a.vals) dot += (double)v_l * (double)mine;
@@ -484,7 +656,7 @@ int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp, const double* la
     a.m = m_out;
     a.vals = vals_dot;
     a.nnz = (uint32_t)c->nnz;
-    const unsigned nb = 2048;
+    const unsigned nb = 4096;
     double* part = nullptr;
     if (vals_dot) {
         void* sp = nullptr;
@@ -516,7 +688,7 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
                     float* const* grad, double* loss_out) {
     const void* Yin[kMaxModes];
     for (int n = 0; n < c->N; ++n) Yin[n] = A[n];
-    double* dloss = nullptr;
+    double* dloss_dev = nullptr;
     for (int mode = 0; mode < c->N; ++mode) {
         SegArgs a;
         int NO = 0;
@@ -526,9 +698,9 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         a.loss = loss;
         if (mode == 0 && loss_out) {
             const size_t nwg = std::max<uint32_t>(a.nwg, 1);
-            a.loss_part = (double*)((char*)a.brow + nwg * 2 * sizeof(uint32_t));
-            a.loss_part = (double*)(((uintptr_t)a.loss_part + 7) & ~(uintptr_t)7);
-            dloss = a.loss_part;
+            uintptr_t lp = (uintptr_t)((char*)a.brow + nwg * 2 * sizeof(uint32_t));
+            a.loss_part = (double*)((lp + 7) & ~(uintptr_t)7);
+            dloss_dev = a.loss_part;
         }
         DYN_CHECK(fill_f32(c, grad[mode], (size_t)c->dims[mode] * Rp, 0.0f));
         if (a.nwg == 0) continue;
@@ -538,14 +710,15 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         if (mode == 0 && loss_out) {
             void* sp = nullptr;
             DYN_CHECK(scratch(c, 64, &sp));
-            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, dloss, a.nwg,
-                               (double*)sp);
+            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, dloss_dev,
+                               a.nwg, (double*)sp);
             DYN_CHECK(check_launch(c, "sum_parts"));
             DYN_HIP(c, hipMemcpyAsync(loss_out, sp, sizeof(double), hipMemcpyDeviceToHost,
                                       c->stream));
             DYN_HIP(c, hipStreamSynchronize(c->stream));
         }
     }
+    if (loss_out && c->nnz == 0) *loss_out = 0.0;
     return DYN_OK;
 }
 
