@@ -70,8 +70,10 @@ SIGNATURES = {
     "dyn_gpu_destroy": (C.c_int, [_vp]),
     "dyn_gpu_last_error": (C.c_char_p, [_vp]),
     "dyn_gpu_set_stream": (C.c_int, [_vp, _vp]),
+    "dyn_gpu_reset_stream": (C.c_int, [_vp]),
     "dyn_gpu_synchronize": (C.c_int, [_vp]),
     "dyn_gpu_mem_used": (C.c_int, [_vp, _u64p]),
+    "dyn_gpu_device_memory": (C.c_int, [C.c_int, _u64p, _u64p]),
     "dyn_gpu_upload_coo": (C.c_int, [_vp, C.c_int, C.c_uint64, _u32p, _f32p, C.POINTER(_u32p)]),
     "dyn_gpu_remap": (C.c_int, [_vp, C.c_int, C.c_int, _u64p, _u32p]),
     "dyn_gpu_remap_all": (C.c_int, [_vp]),

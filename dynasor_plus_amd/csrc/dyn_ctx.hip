@@ -247,13 +247,34 @@ extern "C" const char* dyn_gpu_last_error(const dyn_gpu_ctx* c) {
 
 extern "C" int dyn_gpu_set_stream(dyn_gpu_ctx* c, void* stream) {
     if (!c) return DYN_ERR_INVALID;
-    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    c->stream = (hipStream_t)stream;   // NULL: the device's default stream
+    return DYN_OK;
+}
+
+extern "C" int dyn_gpu_reset_stream(dyn_gpu_ctx* c) {
+    if (!c) return DYN_ERR_INVALID;
+    c->stream = c->own_stream;
     return DYN_OK;
 }
 
 extern "C" int dyn_gpu_synchronize(dyn_gpu_ctx* c) {
     if (!c) return DYN_ERR_INVALID;
     DYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
+extern "C" int dyn_gpu_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return DYN_ERR_HIP;
+    }
+    if (device < 0 || device >= count) return DYN_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return DYN_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return DYN_ERR_HIP;
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
     return DYN_OK;
 }
 

@@ -72,8 +72,26 @@ class GpuContext:
         except Exception:
             pass
 
-    def set_stream(self, stream_ptr: int | None) -> None:
-        self._check(self.lib.dyn_gpu_set_stream(self.h, L._vp(stream_ptr or 0)), "set_stream")
+    def set_stream(self, stream_ptr: int) -> None:
+        """Run on hipStream_t `stream_ptr` (0 = the default stream, torch's default)."""
+        self._check(self.lib.dyn_gpu_set_stream(self.h, L._vp(int(stream_ptr))), "set_stream")
+
+    def reset_stream(self) -> None:
+        self._check(self.lib.dyn_gpu_reset_stream(self.h), "reset_stream")
+
+    def on_stream(self, stream_ptr: int):
+        """Context manager: run on `stream_ptr`, then return to the own stream."""
+        eng = self
+
+        class _On:
+            def __enter__(self):
+                eng.set_stream(stream_ptr)
+                return eng
+
+            def __exit__(self, *exc):
+                eng.reset_stream()
+
+        return _On()
 
     def synchronize(self) -> None:
         self._check(self.lib.dyn_gpu_synchronize(self.h), "synchronize")

@@ -85,12 +85,19 @@ int         dyn_gpu_create(int device, dyn_gpu_ctx** out);
 int         dyn_gpu_destroy(dyn_gpu_ctx* ctx);
 const char* dyn_gpu_last_error(const dyn_gpu_ctx* ctx);
 int         dyn_gpu_abi_version(void);
-/* Run subsequent work on `stream` (a hipStream_t, or NULL for the
- * context's own stream). */
+/* Run subsequent work on `stream`, a hipStream_t; NULL selects the device's
+ * default (null) stream, which is what torch's default stream is.  Callers
+ * that share buffers with another stream must pass that stream here. */
 int         dyn_gpu_set_stream(dyn_gpu_ctx* ctx, void* stream);
+/* Return to the context's own (non-blocking) stream. */
+int         dyn_gpu_reset_stream(dyn_gpu_ctx* ctx);
 int         dyn_gpu_synchronize(dyn_gpu_ctx* ctx);
 /* Bytes of device memory currently held by the context. */
 int         dyn_gpu_mem_used(const dyn_gpu_ctx* ctx, uint64_t* bytes);
+/* Free / total device memory of `device` (no context needed); used by the
+ * planner (dyn_build_compute_plan) the way the reference's planner queries
+ * host RAM (src/dynasor_preprocess.cpp:951-1104). */
+int         dyn_gpu_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* ---- tensor ---------------------------------------------------------- */
 /* Copy a COO tensor (SoA: vals[nnz], idx[n][nnz]) to the device.  Replaces

@@ -64,6 +64,19 @@ typedef struct dyn_gen_spec {
 int dyn_gen_tensor(const dyn_gen_spec* spec, uint32_t* const* idx_out, float* vals_out,
                    double* vals64_out, double* stats_out, int threads);
 
+/* Tensor loaders behind a C-ABI (replace the reference's load_coo_tensor /
+ * save_dnb_tensor, src/dynasor_io.cpp:1184-1261, 854-893, for callers that
+ * cannot use the C++ API).  dyn_host_tensor_load: <path>.dnb when fresh,
+ * else the FROSTT text (then <path>.dnb is written when cache_on_success).
+ * Returns 0 on success, 2 on a parse / format error. */
+typedef struct dyn_host_tensor dyn_host_tensor;
+int  dyn_host_tensor_load(const char* path, int cache_on_success, dyn_host_tensor** out);
+int  dyn_host_tensor_info(const dyn_host_tensor* t, int* num_modes, uint64_t* nnz,
+                          uint32_t* dims);
+int  dyn_host_tensor_copy(const dyn_host_tensor* t, float* vals, uint32_t* const* idx);
+int  dyn_host_tensor_save_dnb(const dyn_host_tensor* t, const char* path);
+void dyn_host_tensor_free(dyn_host_tensor* t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,32 @@ def cpals_fixture(name, tns_path, R, kernel, iters, tol, seed, tmp, extra=None, 
     print(f"cpals_{name}_{kernel}_r{R}: {len(fits)} iterations, last fit {fits[-1] if fits else None}")
 
 
+def c2_fits(tmp):
+    import json
+    T, _, _ = host.generate((12092, 9184, 28818), 77_000_000, seed=2, coords="zipf", alpha=1.1,
+                            values="lognormal")
+    p = Path(tmp) / "c2.tns"
+    tio.save_dnb(str(p) + ".dnb", T)
+    out = run([p, "--decompose", "cpals", "--cp-kernel", "gauss-seidel", "--rank", 32,
+               "--cp-iters", 3, "--cp-tol", 0, "--seed", 42], tmp)
+    fits = [float(m.group(2)) for m in (FIT_RE.match(l) for l in out.splitlines()) if m]
+    rec = {"dims": [12092, 9184, 28818], "nnz": 77_000_000, "seed": 2, "alpha": 1.1,
+           "values": "lognormal", "rank": 32, "init_seed": 42, "checksum": checksum(T),
+           "fits": fits, "threads": "reference default (all host threads)"}
+    # the oracle's trajectories from the same init: FP64 algebra (what the
+    # device computes) and the reference's FP32 algebra
+    sys.path.insert(0, str(ROOT))
+    from tests import _oracle
+    from dynasor_plus_amd import init_factors
+    orc = _oracle.load()
+    Y0 = init_factors(T.dims, 32, 42, "normal")
+    rec["oracle_fp64_fits"] = [float(x) for x in orc.cpals(T, Y0, 3, 0.0, fp64=True)[0]]
+    rec["oracle_fp32_fits"] = [float(x) for x in orc.cpals(T, Y0, 3, 0.0, fp64=False)[0]]
+    rec["oracle_note"] = "oracle/oracle.cpp orc_cpals from the same init (seed 42), fp64=1 and fp64=0"
+    (HERE / "cpals_c2_gauss-seidel_r32.json").write_text(json.dumps(rec, indent=1) + "\n")
+    print(f"cpals_c2: fits {fits}")
+
+
 def main():
     if not REF_BIN.exists():
         sys.exit("build the reference first: make -C oracle ref")
@@ -122,6 +148,10 @@ def main():
         recipe = dict(gen_dims=np.array(T.dims), gen_nnz=T.nnz, gen_seed=1, gen_noise=0.01,
                       gen_rank=10)
         cpals_fixture("c1", p, 16, "gauss-seidel", 50, 1e-4, 42, tmp, extra=recipe, tensor=T)
+        # configuration 2 (12092x9184x28818, 77M distinct Zipf(1.1), lognormal): the
+        # reference's first three Gauss-Seidel fits at rank 32 (stdout only; the
+        # tensor is regenerated by the engine's generator from the recipe)
+        c2_fits(tmp)
 
 
 if __name__ == "__main__":
