@@ -90,7 +90,9 @@ int tensor_norm2(dyn_gpu_ctx* c, double* out) {
 // ---------------------------------------------------------------------------
 struct ScaleArgs {
     float*        Y;          // rows x RP, scaled in place when sq_part != null
-    bf16*         Ybf;        // BF16 shadow to refresh (nullable)
+    float*        Yg;         // gathered FP32 operand, rows in rank order (nullable)
+    bf16*         Ygb;        // gathered BF16 operand (with Yg)
+    const uint32_t* relab;    // row -> rank (with Yg)
     const double* sq_part;    // [sq_nwg][RP] column sums of x^2 from the solve
                               // (nullable: Gram only, no scaling)
     uint32_t      sq_nwg;
@@ -158,7 +160,11 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
                     y = (float)((double)y * inv);
                     a.Y[(size_t)r * RP + col] = y;
                 }
-                if (a.Ybf) a.Ybf[(size_t)r * RP + col] = to_bf16(y);
+                if (a.Yg) {
+                    const size_t g = (size_t)a.relab[r] * RP + col;
+                    a.Yg[g] = y;
+                    a.Ygb[g] = to_bf16(y);
+                }
                 if (a.Yhat) ip += (double)y * (double)a.Yhat[(size_t)r * RP + col];
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y, y, acc, 0, 0, 0);
@@ -219,7 +225,11 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
                     const bool writer = kSplitRows || wave == b;
                     if (writer) {
                         if (scale) a.Y[off] = x;
-                        if (a.Ybf) a.Ybf[off] = to_bf16(x);
+                        if (a.Yg) {
+                            const size_t g = (size_t)a.relab[r] * RP + b * 32 + col;
+                            a.Yg[g] = x;
+                            a.Ygb[g] = to_bf16(x);
+                        }
                         if (a.Yhat) ip[b] += (double)x * (double)a.Yhat[off];
                     }
                 }
@@ -554,20 +564,6 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
     }
 }
 
-__global__ void k_to_bf16(const float* __restrict__ y, bf16* __restrict__ o, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) o[i] = to_bf16(y[i]);
-}
-
-int refresh_bf16(dyn_gpu_ctx* c, int mode) {
-    const size_t n = (size_t)c->dims[mode] * c->Rp;
-    const unsigned nb = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
-    hipLaunchKernelGGL(k_to_bf16, dim3(nb), dim3(kBlock), 0, c->stream, c->Y[mode],
-                       c->Ybf[mode], n);
-    return check_launch(c, "to_bf16");
-}
-
 // ---------------------------------------------------------------------------
 //  host wrappers
 // ---------------------------------------------------------------------------
@@ -585,11 +581,14 @@ struct SolveParts {
 };
 
 template <int RP>
-static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner, bool bf,
+static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner,
                          double** ipart_out, uint32_t* nwg_out) {
     ScaleArgs a;
     a.Y = c->Y[mode];
-    a.Ybf = bf ? c->Ybf[mode] : nullptr;
+    // scaling rewrites Y: the gathered FP32 / BF16 operands follow in the same pass
+    a.Yg = sq ? c->Yg[mode] : nullptr;
+    a.Ygb = sq ? c->Ygb[mode] : nullptr;
+    a.relab = sq ? c->relab[mode] : nullptr;
     a.sq_part = sq ? sq->part : nullptr;
     a.sq_nwg = sq ? sq->nwg : 0;
     a.R = c->R;
@@ -614,19 +613,19 @@ static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool in
     return DYN_OK;
 }
 
-static int scale_gram(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner, bool bf,
+static int scale_gram(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool inner,
                       double** ipart_out, uint32_t* nwg_out) {
     switch (c->Rp) {
-        case 16: return scale_gram_rp<16>(c, mode, sq, inner, bf, ipart_out, nwg_out);
-        case 32: return scale_gram_rp<32>(c, mode, sq, inner, bf, ipart_out, nwg_out);
-        case 64: return scale_gram_rp<64>(c, mode, sq, inner, bf, ipart_out, nwg_out);
-        case 128: return scale_gram_rp<128>(c, mode, sq, inner, bf, ipart_out, nwg_out);
+        case 16: return scale_gram_rp<16>(c, mode, sq, inner, ipart_out, nwg_out);
+        case 32: return scale_gram_rp<32>(c, mode, sq, inner, ipart_out, nwg_out);
+        case 64: return scale_gram_rp<64>(c, mode, sq, inner, ipart_out, nwg_out);
+        case 128: return scale_gram_rp<128>(c, mode, sq, inner, ipart_out, nwg_out);
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", c->Rp);
     }
 }
 
 int gram_mode(dyn_gpu_ctx* c, int mode) {
-    return scale_gram(c, mode, nullptr, false, false, nullptr, nullptr);
+    return scale_gram(c, mode, nullptr, false, nullptr, nullptr);
 }
 
 // V_n^{-1} into c->L on `stream`
@@ -725,7 +724,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     DYN_CHECK(tensor_norm2(c, &tn2));
     res->tensor_norm = std::sqrt(tn2);
 
-    // initial Grams, lambda = 1, BF16 shadows
+    // initial Grams, lambda = 1, gathered operands
     for (int m = 0; m < N; ++m) DYN_CHECK(gram_mode(c, m));
     {
         std::vector<double> ones(c->Rp, 0.0);
@@ -733,8 +732,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         DYN_HIP(c, hipMemcpyAsync(c->lambda, ones.data(), c->Rp * sizeof(double),
                                   hipMemcpyHostToDevice, c->stream));
     }
-    if (bf)
-        for (int m = 0; m < N; ++m) DYN_CHECK(refresh_bf16(c, m));
+    DYN_CHECK(ensure_gather(c));
     DYN_HIP(c, hipStreamSynchronize(c->stream));
 
     if (o->verbose) {
@@ -764,7 +762,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     for (int iter = 0; iter < o->max_iters && rc == DYN_OK; ++iter) {
         auto t0 = clk::now();
         const void* Yin[kMaxModes];
-        for (int m = 0; m < N; ++m) Yin[m] = bf ? (const void*)c->Ybf[m] : (const void*)c->Y[m];
+        for (int m = 0; m < N; ++m) Yin[m] = bf ? (const void*)c->Ygb[m] : (const void*)c->Yg[m];
         int nev = 0;
         if (jacobi) {
             (void)hipEventRecord(ev[0], c->stream);
@@ -801,7 +799,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
             SolveParts sq;
             if ((rc = solve(c, n, &sq)) != DYN_OK) break;
             const bool last = (n == N - 1) && !jacobi;
-            rc = scale_gram(c, n, &sq, last, bf, last ? &ipart : nullptr, last ? &inwg : nullptr);
+            rc = scale_gram(c, n, &sq, last, last ? &ipart : nullptr, last ? &inwg : nullptr);
         }
         if (rc != DYN_OK) break;
         if (jacobi) {
@@ -849,7 +847,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     destroy_events();
     res->total_time_s = secs(t_start, clk::now());
     res->linalg_time_s = std::max(0.0, res->total_time_s - res->mttkrp_time_s);
-    c->bf16_valid = bf;
+    c->gather_valid = (rc == DYN_OK);
     if (rc == DYN_OK && o->verbose) {
         std::printf("[cpals] done  iters=%d  final_fit=%.6f  resid=%.3e\n", res->iters_done,
                     res->final_fit, res->final_residual);
@@ -897,8 +895,9 @@ extern "C" int dyn_gpu_adjoint(dyn_gpu_ctx* c, const float* g, int mode, float* 
     if (c->nnz)
         DYN_HIP(c, hipMemcpyAsync(gd, g, c->nnz * sizeof(float), hipMemcpyHostToDevice,
                                   c->stream));
+    DYN_CHECK(ensure_gather(c));
     const void* Yin[kMaxModes];
-    for (int m = 0; m < c->N; ++m) Yin[m] = c->Y[m];
+    for (int m = 0; m < c->N; ++m) Yin[m] = c->Yg[m];
     DYN_CHECK(mttkrp_mode(c, mode, DYN_PREC_FP32, Yin, gd, od, c->Rp));
     std::vector<float> pad(rows * c->Rp);
     DYN_HIP(c, hipMemcpyAsync(pad.data(), od, pad.size() * sizeof(float), hipMemcpyDeviceToHost,
@@ -926,9 +925,7 @@ extern "C" int dyn_gpu_adjoint_dev(dyn_gpu_ctx* c, const float* g, int mode,
     if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
     DYN_HIP(c, hipSetDevice(c->device));
     if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
-    const void* Yin[kMaxModes];
-    for (int m = 0; m < c->N; ++m) Yin[m] = A[m];
-    return mttkrp_mode(c, mode, DYN_PREC_FP32, Yin, g, grad, ld);
+    return adjoint_mode(c, mode, A, ld, g, grad);
 }
 
 extern "C" int dyn_gpu_completion_grad_dev(dyn_gpu_ctx* c, int loss, const float* const* A,

@@ -129,11 +129,13 @@ __global__ void k_count_oob(const uint32_t* __restrict__ ix, uint64_t n, uint32_
 static void free_factors(dyn_gpu_ctx* c) {
     for (int n = 0; n < kMaxModes; ++n) {
         dfree(c, c->Y[n]);
-        dfree(c, c->Ybf[n]);
+        dfree(c, c->Yg[n]);
+        dfree(c, c->Ygb[n]);
         dfree(c, c->Yhat[n]);
         dfree(c, c->G[n]);
         c->Y[n] = nullptr;
-        c->Ybf[n] = nullptr;
+        c->Yg[n] = nullptr;
+        c->Ygb[n] = nullptr;
         c->Yhat[n] = nullptr;
         c->G[n] = nullptr;
     }
@@ -143,7 +145,7 @@ static void free_factors(dyn_gpu_ctx* c) {
     dfree(c, c->fitbuf);
     c->lambda = c->inv_lambda = c->L = c->fitbuf = nullptr;
     c->R = c->Rp = 0;
-    c->bf16_valid = false;
+    c->gather_valid = false;
 }
 
 void free_copy(dyn_gpu_ctx* c, int m) {
@@ -151,13 +153,32 @@ void free_copy(dyn_gpu_ctx* c, int m) {
     dfree(c, cp.vals);
     for (int w = 0; w < kMaxModes; ++w) dfree(c, cp.idx[w]);
     dfree(c, cp.rowptr);
-    dfree(c, cp.seg);
     dfree(c, cp.perm);
+    dfree(c, cp.seg_k);
+    dfree(c, cp.seg_e);
+    dfree(c, cp.seg_row);
+    dfree(c, cp.pc_k);
+    dfree(c, cp.pc_e);
+    dfree(c, cp.pc_row);
+    dfree(c, cp.heavy_row);
+    dfree(c, cp.heavy_first);
+    dfree(c, cp.heavy_count);
     cp = ModeCopy();
+}
+
+void free_relabel(dyn_gpu_ctx* c) {
+    for (int n = 0; n < kMaxModes; ++n) {
+        dfree(c, c->relab[n]);
+        c->relab[n] = nullptr;
+        c->relab_inv_host[n].clear();
+    }
+    c->relabeled = false;
+    c->gather_valid = false;
 }
 
 static void free_tensor(dyn_gpu_ctx* c) {
     for (int m = 0; m < kMaxModes; ++m) free_copy(c, m);
+    free_relabel(c);
     dfree(c, c->vals);
     c->vals = nullptr;
     for (int n = 0; n < kMaxModes; ++n) {
@@ -200,6 +221,13 @@ extern "C" int dyn_gpu_create(int device, dyn_gpu_ctx** out) {
         (void)hipStreamDestroy(c->own_stream);
         delete c;
         return DYN_ERR_HIP;
+    }
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+            if (prop.multiProcessorCount > 0) c->num_cus = prop.multiProcessorCount;
+            if (prop.sharedMemPerBlock > 0) c->max_lds = prop.sharedMemPerBlock;
+        }
     }
     if (const char* cap = std::getenv("DYN_GPU_MEM_CAP_MB")) {
         long long mb = std::atoll(cap);
@@ -408,6 +436,14 @@ extern "C" int dyn_gpu_get_copy(dyn_gpu_ctx* c, int mode, float* vals_out,
         }
     }
     DYN_HIP(c, hipStreamSynchronize(c->stream));
+    if (idx_out) {
+        // stored columns hold frequency ranks: back to the input indices
+        for (int w = 0; w < c->N; ++w) {
+            if (w == mode || !idx_out[w]) continue;
+            const std::vector<uint32_t>& inv = c->relab_inv_host[w];
+            for (uint64_t k = 0; k < c->nnz; ++k) idx_out[w][k] = inv[idx_out[w][k]];
+        }
+    }
     return DYN_OK;
 }
 
@@ -431,7 +467,8 @@ extern "C" int dyn_gpu_set_factors(dyn_gpu_ctx* c, int rank, const float* const*
         for (int n = 0; n < c->N; ++n) {
             const size_t cnt = (size_t)c->dims[n] * Rp;
             DYN_CHECK(dalloc_t(c, &c->Y[n], cnt));
-            DYN_CHECK(dalloc_t(c, &c->Ybf[n], cnt));
+            DYN_CHECK(dalloc_t(c, &c->Yg[n], cnt));
+            DYN_CHECK(dalloc_t(c, &c->Ygb[n], cnt));
             DYN_CHECK(dalloc_t(c, &c->Yhat[n], cnt));
             DYN_CHECK(dalloc_t(c, &c->G[n], (size_t)Rp * Rp));
         }
@@ -464,7 +501,7 @@ extern "C" int dyn_gpu_set_factors(dyn_gpu_ctx* c, int rank, const float* const*
         DYN_HIP(c, hipMemcpyAsync(c->inv_lambda, ones.data(), Rp * sizeof(double),
                                   hipMemcpyHostToDevice, c->stream));
     }
-    c->bf16_valid = false;
+    c->gather_valid = false;
     DYN_HIP(c, hipStreamSynchronize(c->stream));
     return DYN_OK;
 }
@@ -508,14 +545,9 @@ static int copy_out_rows(dyn_gpu_ctx* c, const float* dsrc, size_t rows, int Rp,
 }
 
 static int gathered_operands(dyn_gpu_ctx* c, int prec, const void** Yin) {
-    for (int w = 0; w < c->N; ++w) Yin[w] = c->Y[w];
-    if (prec == DYN_PREC_BF16) {
-        if (!c->bf16_valid) {
-            for (int w = 0; w < c->N; ++w) DYN_CHECK(refresh_bf16(c, w));
-            c->bf16_valid = true;
-        }
-        for (int w = 0; w < c->N; ++w) Yin[w] = c->Ybf[w];
-    }
+    DYN_CHECK(ensure_gather(c));
+    for (int w = 0; w < c->N; ++w)
+        Yin[w] = (prec == DYN_PREC_BF16) ? (const void*)c->Ygb[w] : (const void*)c->Yg[w];
     return DYN_OK;
 }
 

@@ -18,7 +18,6 @@ namespace dyn {
 
 constexpr int kMaxModes = DYN_GPU_MAX_MODES;
 constexpr int kBlock = 256;          // threads per workgroup for the streaming kernels
-constexpr int kSeg = 64;             // nonzeros per lane group in the segmented spMTTKRP
 constexpr int kMaxRp = DYN_GPU_MAX_RANK;
 
 typedef __hip_bfloat16 bf16;
@@ -32,16 +31,42 @@ inline int padded_rank(int R) {
 
 // One mode's sorted copy: the nonzeros ordered by idx[mode] (ties in input
 // order, or by a secondary key), with that mode's index column dropped and
-// replaced by a row pointer.
+// replaced by a row pointer.  The other modes' index columns hold their
+// frequency ranks (relab[w][idx]) so hot factor rows are a contiguous prefix.
+//
+// Work tables (built on the host from the row pointer):
+//   * a heavy row (> kHeavyMin nonzeros) is cut into pieces of kPiece
+//     nonzeros; piece p covers [pc_k[p], pc_e[p]) of row pc_row[p] and writes
+//     a partial row to slot p; the pieces of one heavy row are consecutive
+//     (heavy_first / heavy_count) and are summed in slot order;
+//   * light rows are never split: consecutive light rows whose first nonzero
+//     falls in the same kLightWindow-aligned window form one segment
+//     [seg_k[i], seg_e[i]) starting at row seg_row[i], stored directly.
 struct ModeCopy {
     bool      built = false;
     int       secondary = -1;
     float*    vals = nullptr;                   // nnz
-    uint32_t* idx[kMaxModes] = {nullptr};      // idx[w] for w != mode
+    uint32_t* idx[kMaxModes] = {nullptr};      // relab[w][idx[w]] for w != mode
     uint32_t* rowptr = nullptr;                 // I_mode + 1
-    uint32_t* seg = nullptr;                    // ceil(nnz / kSeg) + 1 : row holding position s*kSeg
     uint32_t* perm = nullptr;                   // nnz : sorted position -> input position
+    uint32_t  nseg = 0;                         // light segments
+    uint32_t* seg_k = nullptr;
+    uint32_t* seg_e = nullptr;
+    uint32_t* seg_row = nullptr;
+    uint32_t  npieces = 0;                      // heavy-row pieces
+    uint32_t* pc_k = nullptr;
+    uint32_t* pc_e = nullptr;
+    uint32_t* pc_row = nullptr;
+    uint32_t  nheavy = 0;
+    uint32_t* heavy_row = nullptr;              // nheavy
+    uint32_t* heavy_first = nullptr;            // nheavy : first piece
+    uint32_t* heavy_count = nullptr;            // nheavy : pieces
 };
+
+constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a piece
+constexpr uint32_t kPiece = 64 * kPieceLane;         // nonzeros per heavy piece (one wave)
+constexpr uint32_t kHeavyMin = 256;                  // longer rows are heavy
+constexpr uint32_t kLightWindow = 128;
 
 struct TimedPair {
     hipEvent_t a, b;
@@ -74,7 +99,9 @@ struct dyn_gpu_ctx {
     int          R = 0;
     int          Rp = 0;
     float*       Y[dyn::kMaxModes] = {nullptr};
-    dyn::bf16*   Ybf[dyn::kMaxModes] = {nullptr};
+    // gathered operands, rows in frequency-rank order: Yg[w][relab[w][i]] = Y[w][i]
+    float*       Yg[dyn::kMaxModes] = {nullptr};
+    dyn::bf16*   Ygb[dyn::kMaxModes] = {nullptr};
     float*       Yhat[dyn::kMaxModes] = {nullptr};
     double*      G[dyn::kMaxModes] = {nullptr};   // Rp x Rp FP64 Grams
     double*      lambda = nullptr;                // Rp
@@ -82,7 +109,16 @@ struct dyn_gpu_ctx {
     double*      L = nullptr;                     // Rp x Rp inverse of the Hadamard Gram
     double*      fitbuf = nullptr;                // device scalars of the fit
     double*      fitbuf_host = nullptr;           // pinned mirror
-    bool         bf16_valid = false;              // Ybf mirrors Y
+    bool         gather_valid = false;            // Yg / Ygb mirror Y
+
+    // frequency relabelling of every mode (rank 0 = most frequent index)
+    bool         relabeled = false;
+    uint32_t*    relab[dyn::kMaxModes] = {nullptr};
+    std::vector<uint32_t> relab_inv_host[dyn::kMaxModes];
+
+    // device properties
+    int          num_cus = 256;
+    size_t       max_lds = 65536;
 
     // scratch (grown on demand)
     void*        scratch = nullptr;
@@ -139,6 +175,11 @@ void free_copy(dyn_gpu_ctx* c, int mode);
 // the copy's values with vals_override[perm[k]] when non-null.
 int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
                 const float* vals_override, float* out, int Rp);
+// Adjoint of the forward model for mode `mode`: the spMTTKRP of the
+// per-nonzero vector g (input order) with caller factors A (input order,
+// Rp-strided device rows), into out (I_mode x Rp).
+int adjoint_mode(dyn_gpu_ctx* c, int mode, const float* const* A, int Rp, const float* g,
+                 float* out);
 
 int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp,
                  const double* lambda_or_null, float* m_out, const float* vals_dot,
@@ -148,7 +189,13 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
 
 int tensor_norm2(dyn_gpu_ctx* c, double* out);
 int gram_mode(dyn_gpu_ctx* c, int mode);
-int refresh_bf16(dyn_gpu_ctx* c, int mode);
+int ensure_relabel(dyn_gpu_ctx* c);
+// Yg / Ygb of every mode from Y (after set_factors)
+int ensure_gather(dyn_gpu_ctx* c);
+// dst32[relab[i]] = src[i] (and dstbf, if non-null), rows x Rp
+int relabel_rows(dyn_gpu_ctx* c, const float* src, const uint32_t* relab, float* dst32,
+                 bf16* dstbf, size_t rows, int Rp);
+void free_relabel(dyn_gpu_ctx* c);
 int fill_f32(dyn_gpu_ctx* c, float* p, size_t n, float v);
 
 void timing_begin(dyn_gpu_ctx* c, TimedPair* tp);

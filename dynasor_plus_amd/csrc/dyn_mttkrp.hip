@@ -13,18 +13,20 @@
 //     of the row, loaded 16 bytes at a time (4 FP32 or 8 BF16 columns per
 //     load, widened to FP32), so one quad reads a whole factor row per
 //     operand and the per-nonzero bookkeeping is paid by 4 lanes only,
-//   * each quad walks kSeg consecutive nonzeros of the sorted copy; the value
-//     and index columns are loaded 4 at a time (one element per lane,
-//     contiguous) and broadcast inside the quad with DPP quad permutes; the
-//     gathers of a batch are issued before its arithmetic,
-//   * the output row is accumulated in registers while the row pointer says
-//     the row continues; a row that starts and ends inside the quad's range
-//     is stored directly,
-//   * rows cut by a quad boundary leave a partial in LDS; one workgroup-level
-//     pass sums partials of the same row in quad order; rows cut by a
-//     workgroup boundary leave a partial in a global table that a fix-up
-//     kernel sums in workgroup order.  No atomics: results are bitwise
-//     reproducible run to run.
+//   * persistent workgroups (one per CU) first drain the heavy-row pieces,
+//     one wave per piece: the wave loads the piece's kPiece values and index
+//     columns with coalesced loads (kPieceLane per lane, all in flight at
+//     once), quad g then takes the 4 nonzeros held by its own lanes in each
+//     64-nonzero round (DPP quad broadcasts), and the 16 quad partial rows are
+//     summed across the wave into the piece's slot.  A second kernel sums
+//     each heavy row's slots in slot order,
+//   * then the light segments, one quad each, walked in batches of 4 (next
+//     batch prefetched), rows stored as they complete,
+//   * factor rows are gathered by frequency rank: the hottest rows of the
+//     gathered operands are staged in LDS once per workgroup, the rest come
+//     from L2 / HBM; the secondary sort mode's row is reused from registers
+//     while consecutive nonzeros of a quad share it.
+//   No atomics on data: results are bitwise reproducible run to run.
 #include "dyn_internal.hpp"
 
 #include <algorithm>
@@ -32,25 +34,30 @@
 namespace dyn {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kQuads = kBlock / 4;            // quads per workgroup
-constexpr uint32_t kChunk = (uint32_t)kQuads * kSeg;   // nonzeros per workgroup
+constexpr int kQuads = kBlock / 4;            // quads per 256-thread block
 
 struct SegArgs {
     const float*    vals;         // copy values, or the override vector
     const uint32_t* vperm;        // non-null: value of position k is vals[vperm[k]]
-    const uint32_t* oidx[kMaxModes - 1];
-    const void*     Yo[kMaxModes - 1];
+    const uint32_t* oidx[kMaxModes - 1];   // frequency-ranked indices of the other modes
+    const void*     Yo[kMaxModes - 1];     // gathered operands, rows in rank order
+    uint32_t        hot[kMaxModes - 1];    // leading rows of each operand staged in LDS
     const uint32_t* rowptr;
-    const uint32_t* seg;
+    const uint32_t* seg_k;        // light segments
+    const uint32_t* seg_e;
+    const uint32_t* seg_row;
+    uint32_t        nseg;
+    const uint32_t* pc_k;         // heavy-row pieces
+    const uint32_t* pc_e;
+    const uint32_t* pc_row;
+    uint32_t        npieces;
     float*          out;
-    float*          bpart;        // [nwg][2][RP]
-    uint32_t*       brow;         // [nwg][2]
+    float*          part;         // [npieces][RP]
+    double*         loss_dev;     // completion: summed loss (device scalar)
     // completion kernel only
-    const float*    An;           // output-mode factor rows (FP32)
-    double*         loss_part;    // [nwg] or null
+    const float*    An;           // output-mode factor rows (FP32, input order)
+    double*         loss_part;    // [npieces + nseg] or null
     int             loss;
-    uint32_t        nnz;
-    uint32_t        nwg;
 };
 
 // ---------------------------------------------------------------------------
@@ -128,16 +135,6 @@ __device__ __forceinline__ float quad_sum(float x) {
     return x;
 }
 
-// Hardware dispatches workgroup i to XCD i % 8; hand each XCD a contiguous
-// range of chunks so neighbouring chunks (which share boundary rows and
-// row-pointer lines) stay in one L2.
-__device__ __forceinline__ uint32_t xcd_chunk(uint32_t bid, uint32_t nb) {
-    const uint32_t nx = 8;
-    const uint32_t per = nb / nx, rem = nb % nx;
-    const uint32_t x = bid % nx, i = bid / nx;
-    return x * per + min(x, rem) + i;
-}
-
 __device__ __forceinline__ float dloss(int kind, float m, float y) {
     if (kind == DYN_LOSS_BERNOULLI) return 1.0f / (1.0f + __expf(-m)) - y;
     return m - y;
@@ -152,11 +149,25 @@ __device__ __forceinline__ double lossval(int kind, float m, float y) {
     return 0.5 * d * d;
 }
 
-// nonzeros of a 4-batch whose gathers are issued together
-template <int RP, int NO>
-constexpr int batch_hoist() {
-    constexpr int regs = NO * (RP / 4);
-    return regs >= 64 ? 1 : (regs >= 32 ? 2 : 4);
+// Register plan of the segment kernel.  A quad holds per gathered row CPL =
+// RP/4 floats per lane; HB nonzeros have their gathers in flight together; a
+// piece's stream takes kPieceLane * (NO + 1) registers, its broadcast round
+// 4 * (NO + 1); the completion kernel adds the output row and the FP64 loss.
+// The workgroup is as wide as the register file allows (16 waves need <= 128
+// VGPRs per lane, 8 waves <= 256; 4 waves only add AGPR spill room).
+template <int RP, int NO, int KIND>
+constexpr int seg_regs(int hb) {
+    return hb * NO * (RP / 4) + (2 + KIND) * (RP / 4) + (int)(kPieceLane + 4) * (NO + 1) + 24 +
+           64 * KIND;
+}
+template <int RP, int NO, int KIND>
+constexpr int seg_threads() {
+    return seg_regs<RP, NO, KIND>(1) <= 116 ? 1024 : (seg_regs<RP, NO, KIND>(1) <= 240 ? 512 : 256);
+}
+template <int RP, int NO, int KIND>
+constexpr int seg_hoist() {
+    constexpr int budget = seg_threads<RP, NO, KIND>() == 1024 ? 128 : 240;
+    return seg_regs<RP, NO, KIND>(4) <= budget ? 4 : (seg_regs<RP, NO, KIND>(2) <= budget ? 2 : 1);
 }
 
 template <int J, int NO>
@@ -166,103 +177,221 @@ __device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, floa
     for (int w = 0; w < NO; ++w) iw[w] = qb<J>(i_l[w]);
 }
 
+// a factor row by frequency rank: from LDS when hot, else from memory
+template <int RP, typename FT>
+__device__ __forceinline__ void gather_row(const FT* __restrict__ g, const FT* lds_tab,
+                                           uint32_t hot, uint32_t i, int q, float* v) {
+    if (i < hot) load_row<RP, FT>(lds_tab + (size_t)i * RP, q, v);
+    else load_row<RP, FT>(g + (size_t)i * RP, q, v);
+}
+
+// One nonzero's contribution to the quad's accumulator (y[w] = gathered rows).
+// KIND 0 forms v*y0, times the middle operands, then a fused multiply-add
+// with the last operand; KIND 1 forms the model value m = <prod_w y_w, a_row>,
+// the loss derivative g(m, v), and accumulates g * prod_w y_w.
+template <int CPL, int NO, int KIND>
+__device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)[CPL],
+                                           const float* arow, int loss, double* lsum, int q,
+                                           bool want_loss) {
+    if constexpr (KIND == 0) {
+        if constexpr (NO == 1) {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(v, y[0][c], acc[c]);
+        } else {
+            float p[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) p[c] = v * y[0][c];
+#pragma unroll
+            for (int w = 1; w < NO - 1; ++w)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(p[c], y[NO - 1][c], acc[c]);
+        }
+    } else {
+        float p[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) p[c] = y[0][c];
+#pragma unroll
+        for (int w = 1; w < NO; ++w)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
+        float part = 0.f;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) part = __fmaf_rn(p[c], arow[c], part);
+        const float m = quad_sum(part);
+        const float gr = dloss(loss, m, v);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(gr, p[c], acc[c]);
+        if (want_loss && q == 0) *lsum += lossval(loss, m, v);
+    }
+}
+
+// Gathers of HB broadcast nonzeros (operand 0 reused from y0prev when the
+// index repeats), then their accumulation in order.  `live(j)` says whether
+// nonzero j of the batch exists; `before_step(j)` runs ahead of its
+// accumulation (row changes of light segments).
+template <int RP, int NO, typename FT, int KIND, int HB, int CPL, typename Live, typename Before>
+__device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const uint32_t* off,
+                                       const float* vb, const uint32_t (*ib)[NO], int q,
+                                       float* acc, const float* arow, double* lsum,
+                                       uint32_t& prev0, float* y0prev, Live live,
+                                       Before before_step) {
+#pragma unroll
+    for (int j0 = 0; j0 < 4; j0 += HB) {
+        float y[HB][NO][CPL];
+        bool reuse[HB];
+#pragma unroll
+        for (int h = 0; h < HB; ++h) {
+            const int j = j0 + h;
+            const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
+            reuse[h] = (ib[j][0] == before);
+            if (!reuse[h])
+                gather_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[0]), lds + (size_t)off[0] * RP,
+                                   a.hot[0], ib[j][0], q, y[h][0]);
+#pragma unroll
+            for (int w = 1; w < NO; ++w)
+                gather_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]), lds + (size_t)off[w] * RP,
+                                   a.hot[w], ib[j][w], q, y[h][w]);
+        }
+#pragma unroll
+        for (int h = 0; h < HB; ++h) {
+            const int j = j0 + h;
+            if (live(j)) {
+                if (reuse[h]) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) y[h][0][c] = y0prev[c];
+                }
+                before_step(j);
+                accumulate<CPL, NO, KIND>(acc, vb[j], y[h], arow, a.loss, lsum, q,
+                                          a.loss_part != nullptr);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) y0prev[c] = y[h][0][c];
+                prev0 = ib[j][0];
+            }
+        }
+    }
+}
+
 // KIND 0: spMTTKRP / adjoint.  KIND 1: fused completion loss + gradient.
 template <int RP, int NO, typename FT, int KIND>
-__global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
+__global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a) {
     using C = Cols<RP, FT>;
     constexpr int CPL = C::CPL;
-    constexpr int HB = batch_hoist<RP, NO>();
-    __shared__ float pacc[2 * kQuads][RP];
-    __shared__ uint32_t prow[2 * kQuads];
-    __shared__ double lred[kBlock / 64];
+    constexpr int HB = seg_hoist<RP, NO, KIND>();
+    constexpr int TB = seg_threads<RP, NO, KIND>();
+    constexpr int PL = (int)kPieceLane;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    FT* lds = reinterpret_cast<FT*>(lds_raw);
 
     const int tid = threadIdx.x;
-    const int g = tid >> 2;
-    const int q = tid & 3;
-    const uint32_t wg = xcd_chunk(blockIdx.x, a.nwg);
-    const uint32_t wg_s = wg * kChunk;
-    const uint32_t wg_e = min(wg_s + kChunk, a.nnz);
+    const int lane = tid & 63;
+    const int qw = lane >> 2;         // quad within the wave
+    const int q = lane & 3;
 
-    if (tid < 2 * kQuads) prow[tid] = kNone;
-    if (tid < 2) a.brow[2 * wg + tid] = kNone;
+    // stage the hot rows of every operand (16-byte copies)
+    uint32_t off[NO];
+    {
+        uint32_t o = 0;
+#pragma unroll
+        for (int w = 0; w < NO; ++w) {
+            off[w] = o;
+            const uint32_t nvec = a.hot[w] * (uint32_t)(RP * sizeof(FT) / 16);
+            const uint4* src = reinterpret_cast<const uint4*>(a.Yo[w]);
+            uint4* dst = reinterpret_cast<uint4*>(lds + (size_t)o * RP);
+            for (uint32_t e = tid; e < nvec; e += TB) dst[e] = src[e];
+            o += a.hot[w];
+        }
+    }
     __syncthreads();
 
-    const uint32_t s = wg_s + (uint32_t)g * kSeg;
-    const uint32_t e = min(s + (uint32_t)kSeg, a.nnz);
-    double lsum = 0.0;
+    // Static schedule (pieces have equal length but the last of each row, so
+    // round-robin balances them; a shared work counter would serialise on one
+    // L2 atomic per claim).
+    const uint32_t nwaves = gridDim.x * (uint32_t)(TB / 64);
+    const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)(blockIdx.x * (uint32_t)(TB / 64) + (uint32_t)(tid >> 6)));
 
-    if (s < e) {
-        uint32_t row = a.seg[s / kSeg];
-        uint32_t rbeg = a.rowptr[row];
+    // ---- phase 1: heavy-row pieces, one wave each ----
+    for (uint32_t p = gw; p < a.npieces; p += nwaves) {
+        const uint32_t kb = a.pc_k[p];
+        const uint32_t n = a.pc_e[p] - kb;
+        const uint32_t row = a.pc_row[p];
+        // the whole piece's stream: quad g owns nonzeros [16 g, 16 g + 16) of the
+        // piece, lane q of it holds 16 g + 4 t + q for t < kPieceLane
+        float sv[PL];
+        uint32_t si[PL][NO];
+#pragma unroll
+        for (int t = 0; t < PL; ++t) {
+            const uint32_t k = (uint32_t)(16 * qw + 4 * t + q);
+            const bool in = k < n;
+            sv[t] = in ? (a.vperm ? a.vals[a.vperm[kb + k]] : a.vals[kb + k]) : 0.f;
+#pragma unroll
+            for (int w = 0; w < NO; ++w) si[t][w] = in ? a.oidx[w][kb + k] : 0u;
+        }
+        float acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
+        float arow[KIND == 1 ? CPL : 1];
+        if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
+        double lsum = 0.0;
+        uint32_t prev0 = kNone;
+        float y0prev[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
+#pragma unroll
+        for (int t = 0; t < PL; ++t) {
+            const uint32_t k0 = (uint32_t)(16 * qw + 4 * t);
+            if (k0 < n) {
+                float vb[4];
+                uint32_t ib[4][NO];
+                bcast_batch<0, NO>(sv[t], si[t], vb[0], ib[0]);
+                bcast_batch<1, NO>(sv[t], si[t], vb[1], ib[1]);
+                bcast_batch<2, NO>(sv[t], si[t], vb[2], ib[2]);
+                bcast_batch<3, NO>(sv[t], si[t], vb[3], ib[3]);
+                batch4<RP, NO, FT, KIND, HB, CPL>(
+                    a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
+                    [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
+            }
+        }
+        // sum the 16 quad partials (butterfly over quads, fixed order)
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
+        if (qw == 0) store_row<RP, FT>(a.part + (size_t)p * RP, q, acc);
+        if constexpr (KIND == 1) {
+            if (a.loss_part) {
+#pragma unroll
+                for (int o = 4; o < 64; o <<= 1) lsum += __shfl_xor(lsum, o, 64);
+                if (lane == 0) a.loss_part[p] = lsum;
+            }
+        }
+    }
+
+    // ---- phase 2: light segments, one quad each (16 per wave step) ----
+    for (uint32_t base = gw * 16u; base < a.nseg; base += nwaves * 16u) {
+        const uint32_t i = base + (uint32_t)qw;
+        if (i >= a.nseg) continue;
+
+        const uint32_t kb = a.seg_k[i];
+        const uint32_t ke = a.seg_e[i];
+        uint32_t row = a.seg_row[i];
         uint32_t rend = a.rowptr[row + 1];
         float acc[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
         float arow[KIND == 1 ? CPL : 1];
         if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
+        double lsum = 0.0;
+        uint32_t prev0 = kNone;
+        float y0prev[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
 
-        auto flush = [&](uint32_t r, uint32_t rb, uint32_t re) {
-            const bool os = rb < s, oe = re > e;
-            if (!os && !oe) {
-                store_row<RP, FT>(a.out + (size_t)r * RP, q, acc);
-            } else {
-                const int slot = 2 * g + (os ? 0 : 1);
-                store_row<RP, FT>(&pacc[slot][0], q, acc);
-                if (q == 0) prow[slot] = r;
-            }
-        };
-
-        auto step = [&](uint32_t k, float v, const float (*y)[CPL]) {
-            if (k >= rend) {
-                flush(row, rbeg, rend);
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
-                do {
-                    ++row;
-                    rbeg = rend;
-                    rend = a.rowptr[row + 1];
-                } while (k >= rend);
-                if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
-            }
-            if constexpr (KIND == 0) {
-                if constexpr (NO == 1) {
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(v, y[0][c], acc[c]);
-                } else {
-                    float p[CPL];
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) p[c] = v * y[0][c];
-#pragma unroll
-                    for (int w = 1; w < NO - 1; ++w)
-#pragma unroll
-                        for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(p[c], y[NO - 1][c], acc[c]);
-                }
-            } else {
-                float p[CPL];
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) p[c] = y[0][c];
-#pragma unroll
-                for (int w = 1; w < NO; ++w)
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) p[c] *= y[w][c];
-                float part = 0.f;
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) part = __fmaf_rn(p[c], arow[c], part);
-                const float m = quad_sum(part);
-                const float gr = dloss(a.loss, m, v);
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) acc[c] = __fmaf_rn(gr, p[c], acc[c]);
-                if (a.loss_part && q == 0) lsum += lossval(a.loss, m, v);
-            }
-        };
-
-        // value / index columns of the next 4-batch are loaded while the
-        // current batch's gathers are in flight
-        auto fetch = [&](uint32_t kb, float& v_l, uint32_t* i_l) {
-            const uint32_t kk = kb + q;
-            if (kk < e) {
+        auto fetch = [&](uint32_t kx, float& v_l, uint32_t* i_l) {
+            const uint32_t kk = kx + q;
+            if (kk < ke) {
                 v_l = a.vperm ? a.vals[a.vperm[kk]] : a.vals[kk];
 #pragma unroll
                 for (int w = 0; w < NO; ++w) i_l[w] = a.oidx[w][kk];
@@ -274,163 +403,80 @@ __global__ __launch_bounds__(kBlock) void k_seg(SegArgs a) {
         };
         float v_n;
         uint32_t i_n[NO];
-        fetch(s, v_n, i_n);
-        for (uint32_t k0 = s; k0 < e; k0 += 4) {
+        fetch(kb, v_n, i_n);
+        for (uint32_t k0 = kb; k0 < ke; k0 += 4) {
             const float v_l = v_n;
             uint32_t i_l[NO];
 #pragma unroll
             for (int w = 0; w < NO; ++w) i_l[w] = i_n[w];
-            if (k0 + 4 < e) fetch(k0 + 4, v_n, i_n);
-            const uint32_t cnt = min(4u, e - k0);
+            if (k0 + 4 < ke) fetch(k0 + 4, v_n, i_n);
             float vb[4];
             uint32_t ib[4][NO];
             bcast_batch<0, NO>(v_l, i_l, vb[0], ib[0]);
             bcast_batch<1, NO>(v_l, i_l, vb[1], ib[1]);
             bcast_batch<2, NO>(v_l, i_l, vb[2], ib[2]);
             bcast_batch<3, NO>(v_l, i_l, vb[3], ib[3]);
+            batch4<RP, NO, FT, KIND, HB, CPL>(
+                a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
+                [&](int j) { return k0 + (uint32_t)j < ke; },
+                [&](int j) {
+                    const uint32_t k = k0 + (uint32_t)j;
+                    if (k >= rend) {   // the row is complete
+                        store_row<RP, FT>(a.out + (size_t)row * RP, q, acc);
 #pragma unroll
-            for (int j0 = 0; j0 < 4; j0 += HB) {
-                float y[HB][NO][CPL];
-#pragma unroll
-                for (int h = 0; h < HB; ++h)
-#pragma unroll
-                    for (int w = 0; w < NO; ++w)
-                        load_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]) +
-                                             (size_t)ib[j0 + h][w] * RP,
-                                         q, y[h][w]);
-#pragma unroll
-                for (int h = 0; h < HB; ++h)
-                    if ((uint32_t)(j0 + h) < cnt) step(k0 + j0 + h, vb[j0 + h], y[h]);
-            }
+                        for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
+                        do {
+                            ++row;
+                            rend = a.rowptr[row + 1];
+                        } while (k >= rend);
+                        if constexpr (KIND == 1)
+                            load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
+                    }
+                });
         }
-        flush(row, rbeg, rend);
-    }
-    __syncthreads();
-
-    // workgroup-level combine of the partial rows, in quad order
-    for (int half = 0; half < 2; ++half) {
-        const int slot = 2 * g + half;
-        const uint32_t r = prow[slot];
-        if (r == kNone) continue;
-        int p = slot - 1;
-        while (p >= 0 && prow[p] == kNone) --p;
-        if (p >= 0 && prow[p] == r) continue;   // not the first partial of its row
-        float sum[CPL];
-#pragma unroll
-        for (int b = 0; b < C::NB; ++b)
-#pragma unroll
-            for (int t = 0; t < C::LW; ++t) sum[b * C::LW + t] = pacc[slot][C::col(b, q, t)];
-        for (int t2 = slot + 1; t2 < 2 * kQuads; ++t2) {
-            const uint32_t rt = prow[t2];
-            if (rt == kNone) continue;
-            if (rt != r) break;
-#pragma unroll
-            for (int b = 0; b < C::NB; ++b)
-#pragma unroll
-                for (int t = 0; t < C::LW; ++t) sum[b * C::LW + t] += pacc[t2][C::col(b, q, t)];
-        }
-        const bool os = a.rowptr[r] < wg_s;
-        const bool oe = a.rowptr[r + 1] > wg_e;
-        if (!os && !oe) {
-            store_row<RP, FT>(a.out + (size_t)r * RP, q, sum);
-        } else {
-            const int bs = os ? 0 : 1;
-            store_row<RP, FT>(a.bpart + ((size_t)wg * 2 + bs) * RP, q, sum);
-            if (q == 0) a.brow[2 * wg + bs] = r;
-        }
-    }
-
-    if (KIND == 1 && a.loss_part) {
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
-        if ((tid & 63) == 0) lred[tid >> 6] = lsum;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) t += lred[w];
-            a.loss_part[wg] = t;
-        }
+        store_row<RP, FT>(a.out + (size_t)row * RP, q, acc);
+        if (KIND == 1 && a.loss_part && q == 0) a.loss_part[a.npieces + i] = lsum;
     }
 }
 
-// Rows cut by workgroup boundaries: a partial in slot 1 of workgroup b opens
-// the row; slot 0 of workgroups b+1, b+2, ... continue it while the row id
-// matches.  Each quad of a fix-up block owns one workgroup slot; chains of up
-// to kShortChain partials are summed by their quad in order, longer ones
-// (heavy rows) by the whole block: quads stride the chain, then an ordered
-// LDS reduction.  Deterministic.
-constexpr int kShortChain = 8;
-
+// Heavy rows: sum the row's piece partials in slot order (quads stride the
+// slots, then an ordered LDS reduction).  One block per heavy row.
 template <int RP>
-__global__ __launch_bounds__(kBlock) void k_fixup(const float* __restrict__ bpart,
-                                                  const uint32_t* __restrict__ brow,
-                                                  uint32_t nwg, float* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part,
+                                                  const uint32_t* __restrict__ hrow,
+                                                  const uint32_t* __restrict__ hfirst,
+                                                  const uint32_t* __restrict__ hcount,
+                                                  float* __restrict__ out) {
     using C = Cols<RP, float>;
     constexpr int CPL = C::CPL;
     __shared__ float red[kQuads][RP];
-    __shared__ uint32_t longb[kQuads];
-    __shared__ uint32_t nlong;
     const int tid = threadIdx.x;
     const int g = tid >> 2;
     const int q = tid & 3;
-    if (tid == 0) nlong = 0;
-    __syncthreads();
-    const uint32_t b = blockIdx.x * kQuads + g;
-    if (b < nwg) {
-        const uint32_t r = brow[2 * b + 1];
-        if (r != kNone) {
-            float sum[CPL];
-            load_row<RP, float>(bpart + ((size_t)b * 2 + 1) * RP, q, sum);
-            uint32_t c = b + 1;
-            int len = 0;
-            for (; c < nwg && len < kShortChain; ++c, ++len) {
-                if (brow[2 * c] != r) break;
-                float t[CPL];
-                load_row<RP, float>(bpart + ((size_t)c * 2) * RP, q, t);
+    const uint32_t h = blockIdx.x;
+    const uint32_t first = hfirst[h], cnt = hcount[h];
+    float sum[CPL];
 #pragma unroll
-                for (int i = 0; i < CPL; ++i) sum[i] += t[i];
-            }
-            const bool more = (c < nwg) && (len == kShortChain) && (brow[2 * c] == r);
-            if (!more) {
-                store_row<RP, float>(out + (size_t)r * RP, q, sum);
-            } else if (q == 0) {
-                const uint32_t slot = atomicAdd(&nlong, 1u);
-                longb[slot] = b;
-            }
-        }
+    for (int j = 0; j < CPL; ++j) sum[j] = 0.f;
+    for (uint32_t s = g; s < cnt; s += kQuads) {
+        float t[CPL];
+        load_row<RP, float>(part + (size_t)(first + s) * RP, q, t);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) sum[j] += t[j];
     }
+    store_row<RP, float>(&red[g][0], q, sum);
     __syncthreads();
-    const uint32_t nl = nlong;
-    // process long chains in ascending workgroup order (slot order from the
-    // atomic is arbitrary, the arithmetic below does not depend on it)
-    for (uint32_t i = 0; i < nl; ++i) {
-        const uint32_t lb = longb[i];
-        const uint32_t r = brow[2 * lb + 1];
-        float sum[CPL];
+    if (g == 0) {
+        float t[CPL];
+        load_row<RP, float>(&red[0][0], q, t);
+        const int used = (int)min(cnt, (uint32_t)kQuads);
+        for (int j = 1; j < used; ++j) {
+            float u[CPL];
+            load_row<RP, float>(&red[j][0], q, u);
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) sum[j] = 0.f;
-        if (g == 0) load_row<RP, float>(bpart + ((size_t)lb * 2 + 1) * RP, q, sum);
-        for (uint32_t c = lb + 1 + g; c < nwg; c += kQuads) {
-            if (brow[2 * c] != r) break;
-            float t[CPL];
-            load_row<RP, float>(bpart + ((size_t)c * 2) * RP, q, t);
-#pragma unroll
-            for (int j = 0; j < CPL; ++j) sum[j] += t[j];
+            for (int k = 0; k < CPL; ++k) t[k] += u[k];
         }
-        store_row<RP, float>(&red[g][0], q, sum);
-        __syncthreads();
-        if (g == 0) {
-            float t[CPL];
-            load_row<RP, float>(&red[0][0], q, t);
-            for (int j = 1; j < kQuads; ++j) {
-                float u[CPL];
-                load_row<RP, float>(&red[j][0], q, u);
-#pragma unroll
-                for (int k = 0; k < CPL; ++k) t[k] += u[k];
-            }
-            store_row<RP, float>(out + (size_t)r * RP, q, t);
-        }
-        __syncthreads();
+        store_row<RP, float>(out + (size_t)hrow[h] * RP, q, t);
     }
 }
 
@@ -438,51 +484,67 @@ __global__ __launch_bounds__(kBlock) void k_fixup(const float* __restrict__ bpar
 //  host-side dispatch
 // ---------------------------------------------------------------------------
 template <int RP, int NO, typename FT, int KIND>
-static void launch_seg(dyn_gpu_ctx* c, const SegArgs& a) {
-    hipLaunchKernelGGL((k_seg<RP, NO, FT, KIND>), dim3(a.nwg), dim3(kBlock), 0, c->stream, a);
-}
-
-template <int RP, typename FT, int KIND>
-static int dispatch_no(dyn_gpu_ctx* c, int NO, const SegArgs& a) {
-    switch (NO) {
-        case 1: launch_seg<RP, 1, FT, KIND>(c, a); break;
-        case 2: launch_seg<RP, 2, FT, KIND>(c, a); break;
-        case 3: launch_seg<RP, 3, FT, KIND>(c, a); break;
-        case 4: launch_seg<RP, 4, FT, KIND>(c, a); break;
-        case 5: launch_seg<RP, 5, FT, KIND>(c, a); break;
-        case 6: launch_seg<RP, 6, FT, KIND>(c, a); break;
-        case 7: launch_seg<RP, 7, FT, KIND>(c, a); break;
-        default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported mode count %d", NO + 1);
+static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
+    // hot rows: equal share of the LDS budget per gathered operand
+    const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
+    const size_t rowb = (size_t)RP * sizeof(FT);
+    const uint32_t per = (uint32_t)(budget / ((size_t)NO * rowb));
+    size_t bytes = 0;
+    for (int w = 0; w < NO; ++w) {
+        a.hot[w] = std::min<uint32_t>(per, dims_o[w]);
+        bytes += (size_t)a.hot[w] * rowb;
     }
+    auto fn = k_seg<RP, NO, FT, KIND>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)std::min<size_t>(c->max_lds, 160u * 1024u));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()), bytes, c->stream, a);
     return DYN_OK;
 }
 
+template <int RP, typename FT, int KIND>
+static int dispatch_no(dyn_gpu_ctx* c, int NO, SegArgs& a, const uint32_t* d) {
+    switch (NO) {
+        case 1: return launch_seg<RP, 1, FT, KIND>(c, a, d);
+        case 2: return launch_seg<RP, 2, FT, KIND>(c, a, d);
+        case 3: return launch_seg<RP, 3, FT, KIND>(c, a, d);
+        case 4: return launch_seg<RP, 4, FT, KIND>(c, a, d);
+        case 5: return launch_seg<RP, 5, FT, KIND>(c, a, d);
+        case 6: return launch_seg<RP, 6, FT, KIND>(c, a, d);
+        case 7: return launch_seg<RP, 7, FT, KIND>(c, a, d);
+        default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported mode count %d", NO + 1);
+    }
+}
+
 template <typename FT, int KIND>
-static int dispatch_rp(dyn_gpu_ctx* c, int Rp, int NO, const SegArgs& a) {
+static int dispatch_rp(dyn_gpu_ctx* c, int Rp, int NO, SegArgs& a, const uint32_t* d) {
     switch (Rp) {
-        case 16: return dispatch_no<16, FT, KIND>(c, NO, a);
-        case 32: return dispatch_no<32, FT, KIND>(c, NO, a);
-        case 64: return dispatch_no<64, FT, KIND>(c, NO, a);
-        case 128: return dispatch_no<128, FT, KIND>(c, NO, a);
+        case 16: return dispatch_no<16, FT, KIND>(c, NO, a, d);
+        case 32: return dispatch_no<32, FT, KIND>(c, NO, a, d);
+        case 64: return dispatch_no<64, FT, KIND>(c, NO, a, d);
+        case 128: return dispatch_no<128, FT, KIND>(c, NO, a, d);
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", Rp);
     }
 }
 
-static int launch_fixup(dyn_gpu_ctx* c, int Rp, const float* bpart, const uint32_t* brow,
-                        uint32_t nwg, float* out) {
-    const unsigned nb = (nwg + kQuads - 1) / kQuads;
+static int launch_heavy(dyn_gpu_ctx* c, int Rp, const ModeCopy& cp, const float* part,
+                        float* out) {
+    if (cp.nheavy == 0) return DYN_OK;
     switch (Rp) {
-        case 16: hipLaunchKernelGGL(k_fixup<16>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 32: hipLaunchKernelGGL(k_fixup<32>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 64: hipLaunchKernelGGL(k_fixup<64>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
-        case 128: hipLaunchKernelGGL(k_fixup<128>, dim3(nb), dim3(kBlock), 0, c->stream, bpart, brow, nwg, out); break;
+        case 16: hipLaunchKernelGGL(k_heavy<16>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
+        case 32: hipLaunchKernelGGL(k_heavy<32>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
+        case 64: hipLaunchKernelGGL(k_heavy<64>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
+        case 128: hipLaunchKernelGGL(k_heavy<128>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", Rp);
     }
-    return check_launch(c, "fixup");
+    return check_launch(c, "heavy_rows");
 }
 
 static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin, SegArgs* a,
-                       int* NO) {
+                       int* NO, uint32_t* dims_o, bool loss_slots) {
     const ModeCopy& cp = c->copy[mode];
     if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
     *a = SegArgs();
@@ -491,22 +553,29 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
         if (w == mode) continue;
         a->oidx[j] = cp.idx[w];
         a->Yo[j] = Yin[w];
+        dims_o[j] = c->dims[w];
         ++j;
     }
     *NO = j;
     a->vals = cp.vals;
     a->rowptr = cp.rowptr;
-    a->seg = cp.seg;
-    a->nnz = (uint32_t)c->nnz;
-    a->nwg = (uint32_t)((c->nnz + kChunk - 1) / kChunk);
+    a->seg_k = cp.seg_k;
+    a->seg_e = cp.seg_e;
+    a->seg_row = cp.seg_row;
+    a->nseg = cp.nseg;
+    a->pc_k = cp.pc_k;
+    a->pc_e = cp.pc_e;
+    a->pc_row = cp.pc_row;
+    a->npieces = cp.npieces;
     void* sp = nullptr;
-    const size_t nwg = std::max<uint32_t>(a->nwg, 1);
-    DYN_CHECK(scratch2(c, nwg * 2 * (size_t)Rp * sizeof(float) + nwg * 2 * sizeof(uint32_t) +
-                              nwg * sizeof(double) + 1024,
-                       &sp));
-    a->bpart = (float*)sp;
-    a->brow = (uint32_t*)((char*)sp + nwg * 2 * (size_t)Rp * sizeof(float));
-    a->loss_part = nullptr;
+    const size_t pieces = std::max<uint32_t>(cp.npieces, 1);
+    const size_t pbytes = ((pieces * Rp * sizeof(float)) + 255) & ~size_t(255);
+    const size_t nloss = (size_t)cp.npieces + cp.nseg;
+    const size_t lbytes = loss_slots ? ((std::max<size_t>(nloss, 1) * sizeof(double) + 255) & ~size_t(255)) : 0;
+    DYN_CHECK(scratch2(c, pbytes + lbytes + 256, &sp));
+    a->part = (float*)sp;
+    a->loss_part = loss_slots ? (double*)((char*)sp + pbytes) : nullptr;
+    a->loss_dev = (double*)((char*)sp + pbytes + lbytes);
     return DYN_OK;
 }
 
@@ -514,23 +583,55 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
                 const float* vals_override, float* out, int Rp) {
     SegArgs a;
     int NO = 0;
-    DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO));
+    uint32_t dims_o[kMaxModes];
+    DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false));
     if (vals_override) {
         a.vals = vals_override;
         a.vperm = c->copy[mode].perm;
     }
     a.out = out;
     DYN_CHECK(fill_f32(c, out, (size_t)c->dims[mode] * Rp, 0.0f));
-    if (a.nwg == 0) return DYN_OK;
+    if (a.nseg == 0 && a.npieces == 0) return DYN_OK;
     TimedPair tp;
     timing_begin(c, &tp);
     if (prec == DYN_PREC_BF16)
-        DYN_CHECK((dispatch_rp<bf16, 0>(c, Rp, NO, a)));
+        DYN_CHECK((dispatch_rp<bf16, 0>(c, Rp, NO, a, dims_o)));
     else
-        DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a)));
+        DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a, dims_o)));
     DYN_CHECK(check_launch(c, "mttkrp_seg"));
     timing_end(c, &tp);
-    return launch_fixup(c, Rp, a.bpart, a.brow, a.nwg, out);
+    return launch_heavy(c, Rp, c->copy[mode], a.part, out);
+}
+
+// Relabelled copies (rows in frequency-rank order) of caller factors A[n]
+// (input order, Rp-strided), in the first scratch buffer; mode `skip` is not
+// gathered and gets none.
+static int relabeled_operands(dyn_gpu_ctx* c, const float* const* A, int Rp, int skip,
+                              const void** Yin) {
+    DYN_CHECK(ensure_relabel(c));
+    size_t total = 0;
+    for (int n = 0; n < c->N; ++n) total += (size_t)c->dims[n] * Rp;
+    void* sp = nullptr;
+    DYN_CHECK(scratch(c, total * sizeof(float) + 256 * kMaxModes, &sp));
+    char* p = (char*)sp;
+    for (int n = 0; n < c->N; ++n) {
+        if (n == skip) {
+            Yin[n] = nullptr;
+            continue;
+        }
+        float* dst = (float*)p;
+        DYN_CHECK(relabel_rows(c, A[n], c->relab[n], dst, nullptr, c->dims[n], Rp));
+        Yin[n] = dst;
+        p += (((size_t)c->dims[n] * Rp * sizeof(float)) + 255) & ~size_t(255);
+    }
+    return DYN_OK;
+}
+
+int adjoint_mode(dyn_gpu_ctx* c, int mode, const float* const* A, int Rp, const float* g,
+                 float* out) {
+    const void* Yin[kMaxModes];
+    DYN_CHECK(relabeled_operands(c, A, Rp, mode, Yin));
+    return mttkrp_mode(c, mode, DYN_PREC_FP32, Yin, g, out, Rp);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,38 +788,32 @@ int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp, const double* la
 int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
                     float* const* grad, double* loss_out) {
     const void* Yin[kMaxModes];
-    for (int n = 0; n < c->N; ++n) Yin[n] = A[n];
-    double* dloss_dev = nullptr;
+    DYN_CHECK(relabeled_operands(c, A, Rp, -1, Yin));
+    if (loss_out) *loss_out = 0.0;
     for (int mode = 0; mode < c->N; ++mode) {
         SegArgs a;
         int NO = 0;
-        DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO));
+        uint32_t dims_o[kMaxModes];
+        const bool want_loss = (mode == 0 && loss_out);
+        DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, want_loss));
         a.out = grad[mode];
         a.An = A[mode];
         a.loss = loss;
-        if (mode == 0 && loss_out) {
-            const size_t nwg = std::max<uint32_t>(a.nwg, 1);
-            uintptr_t lp = (uintptr_t)((char*)a.brow + nwg * 2 * sizeof(uint32_t));
-            a.loss_part = (double*)((lp + 7) & ~(uintptr_t)7);
-            dloss_dev = a.loss_part;
-        }
         DYN_CHECK(fill_f32(c, grad[mode], (size_t)c->dims[mode] * Rp, 0.0f));
-        if (a.nwg == 0) continue;
-        DYN_CHECK((dispatch_rp<float, 1>(c, Rp, NO, a)));
+        if (a.nseg == 0 && a.npieces == 0) continue;
+        DYN_CHECK((dispatch_rp<float, 1>(c, Rp, NO, a, dims_o)));
         DYN_CHECK(check_launch(c, "completion_seg"));
-        DYN_CHECK(launch_fixup(c, Rp, a.bpart, a.brow, a.nwg, grad[mode]));
-        if (mode == 0 && loss_out) {
-            void* sp = nullptr;
-            DYN_CHECK(scratch(c, 64, &sp));
-            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, dloss_dev,
-                               a.nwg, (double*)sp);
+        DYN_CHECK(launch_heavy(c, Rp, c->copy[mode], a.part, grad[mode]));
+        if (want_loss) {
+            double* d = a.loss_dev;
+            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, a.loss_part,
+                               a.npieces + a.nseg, d);
             DYN_CHECK(check_launch(c, "sum_parts"));
-            DYN_HIP(c, hipMemcpyAsync(loss_out, sp, sizeof(double), hipMemcpyDeviceToHost,
+            DYN_HIP(c, hipMemcpyAsync(loss_out, d, sizeof(double), hipMemcpyDeviceToHost,
                                       c->stream));
             DYN_HIP(c, hipStreamSynchronize(c->stream));
         }
     }
-    if (loss_out && c->nnz == 0) *loss_out = 0.0;
     return DYN_OK;
 }
 

@@ -1,7 +1,8 @@
 // Per-mode remap on the device: stable LSD radix sort of the nonzeros by the
 // output-mode coordinate (optionally with a secondary coordinate as the
-// minor key), gather of the value / other-index columns into the mode copy,
-// and the row pointer plus the kSeg-granular segment table.
+// minor key), gather of the value / other-index columns into the mode copy
+// (other-mode indices replaced by their frequency ranks), the row pointer and
+// the row-segment table.
 //
 // Semantics follow the reference's NCopy + fiber sort + CSR compaction
 // (src/dynasor_preprocess.cpp:471-521 counting sort per shard, :1467-1607
@@ -14,6 +15,8 @@
 #include "dyn_internal.hpp"
 
 #include <algorithm>
+#include <numeric>
+#include <vector>
 
 namespace dyn {
 
@@ -212,14 +215,40 @@ __global__ void k_rowptr(const uint32_t* __restrict__ keys, uint32_t n, uint32_t
     }
 }
 
-// seg[s] = row of sorted position s*kSeg; seg[nseg] = dim
-__global__ void k_segtable(const uint32_t* __restrict__ keys, uint32_t n, uint32_t nseg,
-                           uint32_t dim, uint32_t* __restrict__ seg) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// dst[p] = relab[src[perm[p]]]
+__global__ void k_gather_relabel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm,
+                                 const uint32_t* __restrict__ relab, uint32_t* __restrict__ dst,
+                                 uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t stride = gridDim.x * blockDim.x;
-    for (; s <= nseg; s += stride) {
-        const uint64_t p = (uint64_t)s * kSeg;
-        seg[s] = (s == nseg || p >= n) ? dim : keys[p];
+    for (; i < n; i += stride) dst[i] = relab[src[perm[i]]];
+}
+
+__global__ void k_bincount(const uint32_t* __restrict__ ix, uint32_t n, uint32_t* __restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (; i < n; i += stride) atomicAdd(&cnt[ix[i]], 1u);
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+// dst32[relab[r]] = src[r] (and the BF16 rounding into dstbf), rows x RP
+__global__ void k_relabel_rows(const float* __restrict__ src, const uint32_t* __restrict__ relab,
+                               float* __restrict__ dst32, bf16* __restrict__ dstbf, size_t rows,
+                               int RP) {
+    const size_t n = rows * (size_t)RP;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; e < n; e += stride) {
+        const size_t r = e / RP, col = e % RP;
+        const size_t o = (size_t)relab[r] * RP + col;
+        const float x = src[e];
+        if (dst32) dst32[o] = x;
+        if (dstbf) dstbf[o] = __float2bfloat16(x);
     }
 }
 
@@ -255,15 +284,126 @@ static int radix_passes(dyn_gpu_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1
     return DYN_OK;
 }
 
+// Frequency ranks of every mode's indices: rank 0 is the index with the most
+// nonzeros (ties by index).  The ranking only orders factor rows for the
+// gathers; results do not depend on it.
+int ensure_relabel(dyn_gpu_ctx* c) {
+    if (c->relabeled) return DYN_OK;
+    for (int w = 0; w < c->N; ++w) {
+        const uint32_t dim = c->dims[w];
+        DYN_CHECK(dalloc_t(c, &c->relab[w], dim));
+        void* sp = nullptr;
+        DYN_CHECK(scratch(c, (size_t)dim * sizeof(uint32_t), &sp));
+        uint32_t* cnt = (uint32_t*)sp;
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(dim)), dim3(kBlock), 0, c->stream, cnt,
+                           (size_t)dim);
+        if (c->nnz)
+            hipLaunchKernelGGL(k_bincount, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream,
+                               c->idx[w], (uint32_t)c->nnz, cnt);
+        DYN_CHECK(check_launch(c, "bincount"));
+        std::vector<uint32_t> h(dim);
+        DYN_HIP(c, hipMemcpyAsync(h.data(), cnt, (size_t)dim * 4, hipMemcpyDeviceToHost, c->stream));
+        DYN_HIP(c, hipStreamSynchronize(c->stream));
+        std::vector<uint32_t>& inv = c->relab_inv_host[w];
+        inv.resize(dim);
+        std::iota(inv.begin(), inv.end(), 0u);
+        std::stable_sort(inv.begin(), inv.end(),
+                         [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
+        std::vector<uint32_t> rank(dim);
+        for (uint32_t r = 0; r < dim; ++r) rank[inv[r]] = r;
+        DYN_HIP(c, hipMemcpyAsync(c->relab[w], rank.data(), (size_t)dim * 4,
+                                  hipMemcpyHostToDevice, c->stream));
+        DYN_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->relabeled = true;
+    return DYN_OK;
+}
+
+int relabel_rows(dyn_gpu_ctx* c, const float* src, const uint32_t* relab, float* dst32,
+                 bf16* dstbf, size_t rows, int Rp) {
+    const size_t n = rows * (size_t)Rp;
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_relabel_rows, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, relab,
+                       dst32, dstbf, rows, Rp);
+    return check_launch(c, "relabel_rows");
+}
+
+int ensure_gather(dyn_gpu_ctx* c) {
+    if (c->gather_valid) return DYN_OK;
+    DYN_CHECK(ensure_relabel(c));
+    for (int w = 0; w < c->N; ++w)
+        DYN_CHECK(relabel_rows(c, c->Y[w], c->relab[w], c->Yg[w], c->Ygb[w], c->dims[w], c->Rp));
+    c->gather_valid = true;
+    return DYN_OK;
+}
+
+// Work tables from the row pointer (host): heavy rows cut into kPiece
+// pieces, light rows grouped by the kLightWindow window of their first
+// nonzero.
+static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32_t>& rp) {
+    const uint32_t dim = (uint32_t)rp.size() - 1;
+    std::vector<uint32_t> sk, se, sr, pk, pe, pr, hr, hf, hc;
+    bool open = false;
+    uint32_t window = 0;
+    for (uint32_t r = 0; r < dim; ++r) {
+        const uint32_t L = rp[r + 1] - rp[r];
+        if (L == 0) continue;
+        if (L > kHeavyMin) {
+            if (open) {
+                se.push_back(rp[r]);
+                open = false;
+            }
+            const uint32_t np = (L + kPiece - 1) / kPiece;
+            hr.push_back(r);
+            hf.push_back((uint32_t)pk.size());
+            hc.push_back(np);
+            for (uint32_t j = 0; j < np; ++j) {
+                pk.push_back(rp[r] + j * kPiece);
+                pe.push_back(std::min(rp[r] + (j + 1) * kPiece, rp[r + 1]));
+                pr.push_back(r);
+            }
+        } else {
+            const uint32_t wdw = rp[r] / kLightWindow;
+            if (!open || wdw != window) {
+                if (open) se.push_back(rp[r]);
+                sk.push_back(rp[r]);
+                sr.push_back(r);
+                window = wdw;
+                open = true;
+            }
+        }
+    }
+    if (open) se.push_back(rp[dim]);
+    cp.nseg = (uint32_t)sk.size();
+    cp.npieces = (uint32_t)pk.size();
+    cp.nheavy = (uint32_t)hr.size();
+    auto up = [&](uint32_t** d, const std::vector<uint32_t>& v) -> int {
+        DYN_CHECK(dalloc_t(c, d, std::max<size_t>(v.size(), 1)));
+        if (!v.empty())
+            DYN_HIP(c, hipMemcpyAsync(*d, v.data(), v.size() * 4, hipMemcpyHostToDevice, c->stream));
+        return DYN_OK;
+    };
+    DYN_CHECK(up(&cp.seg_k, sk));
+    DYN_CHECK(up(&cp.seg_e, se));
+    DYN_CHECK(up(&cp.seg_row, sr));
+    DYN_CHECK(up(&cp.pc_k, pk));
+    DYN_CHECK(up(&cp.pc_e, pe));
+    DYN_CHECK(up(&cp.pc_row, pr));
+    DYN_CHECK(up(&cp.heavy_row, hr));
+    DYN_CHECK(up(&cp.heavy_first, hf));
+    DYN_CHECK(up(&cp.heavy_count, hc));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary) {
+    DYN_CHECK(ensure_relabel(c));
     free_copy(c, mode);
     ModeCopy& cp = c->copy[mode];
     const uint32_t n = (uint32_t)c->nnz;
     const uint32_t dim = c->dims[mode];
-    const uint32_t nseg = (uint32_t)((c->nnz + kSeg - 1) / kSeg);
 
     DYN_CHECK(dalloc_t(c, &cp.rowptr, (size_t)dim + 1));
-    DYN_CHECK(dalloc_t(c, &cp.seg, (size_t)nseg + 1));
     DYN_CHECK(dalloc_t(c, &cp.perm, std::max<uint32_t>(n, 1)));
     DYN_CHECK(dalloc_t(c, &cp.vals, std::max<uint32_t>(n, 1)));
     for (int w = 0; w < c->N; ++w)
@@ -311,22 +451,22 @@ int remap_mode(dyn_gpu_ctx* c, int mode, int secondary) {
                            cp.vals, n);
         for (int w = 0; w < c->N; ++w)
             if (w != mode)
-                hipLaunchKernelGGL(k_gather_u32, dim3(g), dim3(kBlock), 0, c->stream, c->idx[w],
-                                   cp.perm, cp.idx[w], n);
+                hipLaunchKernelGGL(k_gather_relabel, dim3(g), dim3(kBlock), 0, c->stream,
+                                   c->idx[w], cp.perm, c->relab[w], cp.idx[w], n);
         hipLaunchKernelGGL(k_rowptr, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
                            keys, n, dim, cp.rowptr);
-        hipLaunchKernelGGL(k_segtable, dim3(grid_for((uint64_t)nseg + 1)), dim3(kBlock), 0,
-                           c->stream, keys, n, nseg, dim, cp.seg);
         DYN_CHECK(check_launch(c, "remap gather/rowptr"));
     } else {
         hipLaunchKernelGGL(k_rowptr, dim3(grid_for((uint64_t)dim + 1)), dim3(kBlock), 0,
                            c->stream, kA, 0u, dim, cp.rowptr);
-        hipLaunchKernelGGL(k_segtable, dim3(1), dim3(kBlock), 0, c->stream, kA, 0u, 0u, dim,
-                           cp.seg);
         DYN_CHECK(check_launch(c, "remap empty"));
     }
+    std::vector<uint32_t> rp((size_t)dim + 1);
+    DYN_HIP(c, hipMemcpyAsync(rp.data(), cp.rowptr, rp.size() * 4, hipMemcpyDeviceToHost,
+                              c->stream));
     // the scratch buffers are reused by the next call: finish before returning
     DYN_HIP(c, hipStreamSynchronize(c->stream));
+    DYN_CHECK(build_segments(c, cp, rp));
     cp.built = true;
     cp.secondary = secondary;
     return DYN_OK;

@@ -151,6 +151,18 @@ double model_norm_sq(const std::vector<std::vector<T>>& G, const T* lambda, int 
     return std::max(0.0, s);
 }
 
+// bfloat16 rounding of an FP32 value (round to nearest, ties to even, on the
+// 16 dropped mantissa bits), returned as the FP32 value it represents.
+float bf16_round(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float y;
+    std::memcpy(&y, &u, 4);
+    return y;
+}
+
 template <typename T>
 int cpals_t(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
             const uint32_t* const* idx, int R, float* const* Yio, const orc_cp_opts* o,
@@ -170,8 +182,16 @@ int cpals_t(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
     std::vector<const T*> Yp(N);
     double fit_prev = 0.0, fit = 0.0;
     int done = 0;
+    std::vector<std::vector<T>> Yb(o->bf16_gather ? N : 0);
     auto mttkrp = [&](int n) {
-        for (int w = 0; w < N; ++w) Yp[w] = Y[w].data();
+        for (int w = 0; w < N; ++w) {
+            Yp[w] = Y[w].data();
+            if (o->bf16_gather && w != n) {
+                Yb[w].resize(Y[w].size());
+                for (size_t e = 0; e < Y[w].size(); ++e) Yb[w][e] = (T)bf16_round((float)Y[w][e]);
+                Yp[w] = Yb[w].data();
+            }
+        }
         mttkrp_def<T>(N, nnz, dims, vals, idx, R, R, Yp.data(), n, Yh[n].data());
     };
     for (int iter = 0; iter < o->max_iters; ++iter) {

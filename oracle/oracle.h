@@ -41,6 +41,9 @@ typedef struct orc_cp_opts {
     int    fp64;               /* 0: FP32 factors and algebra as the reference; 1: FP64 */
     int    stale_jacobi_fit;   /* 1: Jacobi fit from the Yhat identity as the reference
                                   prints it (not the true fit); 0: true fit */
+    int    bf16_gather;        /* 1: the spMTTKRP reads factor rows rounded to bfloat16
+                                  (round to nearest even of the FP32 value): the
+                                  engine's mixed-precision mode */
 } orc_cp_opts;
 
 int orc_mttkrp(int N, uint64_t nnz, const uint32_t* dims, const float* vals,

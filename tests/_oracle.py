@@ -17,7 +17,8 @@ f64p = C.POINTER(C.c_double)
 
 class CpOpts(C.Structure):
     _fields_ = [("max_iters", C.c_int), ("tol", C.c_double), ("ridge", C.c_double),
-                ("jacobi", C.c_int), ("fp64", C.c_int), ("stale_jacobi_fit", C.c_int)]
+                ("jacobi", C.c_int), ("fp64", C.c_int), ("stale_jacobi_fit", C.c_int),
+                ("bf16_gather", C.c_int)]
 
 
 class Oracle:
@@ -74,13 +75,14 @@ class Oracle:
         return perm[: T.nnz], rowptr
 
     def cpals(self, T, Y0, max_iters, tol, ridge=1e-10, jacobi=False, fp64=False,
-              stale_jacobi_fit=False):
+              stale_jacobi_fit=False, bf16_gather=False):
         R = Y0[0].shape[1]
         Y = [np.array(y, dtype=np.float32, copy=True) for y in Y0]
         yarr = (f32p * len(Y))(*[y.ctypes.data_as(f32p) for y in Y])
         cols, carr = self._cols(T)
         dims = np.array(T.dims, dtype=np.uint32)
-        o = CpOpts(max_iters, tol, ridge, int(jacobi), int(fp64), int(stale_jacobi_fit))
+        o = CpOpts(max_iters, tol, ridge, int(jacobi), int(fp64), int(stale_jacobi_fit),
+                   int(bf16_gather))
         hist = np.zeros(max(max_iters, 1), dtype=np.float64)
         lam = np.zeros(R, dtype=np.float64)
         fit = C.c_double(0)

@@ -5,7 +5,8 @@
     permutation) and the spMTTKRP column invariant, plus the reference's
     first three Gauss-Seidel fits (golden, printed to 6 digits);
   * configuration-3 shape (4 modes, Zipf(1.2), rank 64) scaled down so the
-    FP64 oracle finishes: BF16-gathered CP-ALS vs plain vs FP64 oracle;
+    FP64 oracle finishes: plain CP-ALS vs the FP64 oracle, BF16-gathered
+    CP-ALS vs the FP64 oracle with BF16-rounded gathers;
   * configuration-4 shape (5 modes, Zipf(1.0), planted rank 20 + 5% noise,
     rank 32, Jacobi) scaled down: true fit vs the FP64 Jacobi oracle.
 """
@@ -89,14 +90,24 @@ def test_c3_shape_mixed_precision_equal_fit():
     R, iters = 64, 10
     Y0 = init_factors(T.dims, R, 42, "uniform")
     fits64, _, _ = orc.cpals(T, Y0, iters, 0.0, fp64=True)
+    fitsbf, _, _ = orc.cpals(T, Y0, iters, 0.0, fp64=True, bf16_gather=True)
     with GpuContext(0) as ctx:
         ctx.upload(T)
         ctx.set_factors(Y0, R)
         plain = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_FP32)
         ctx.set_factors(Y0, R)
         mixed = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_BF16)
-    assert abs(plain.final_fit - fits64[-1]) < 1e-5, (plain.final_fit, fits64[-1])
-    assert abs(mixed.final_fit - plain.final_fit) < 1e-4, (mixed.final_fit, plain.final_fit)
+    say("fits", list(plain.fit_history), list(mixed.fit_history), fitsbf.tolist())
+    np.testing.assert_allclose(plain.fit_history, fits64, rtol=0, atol=1e-5)
+    # The BF16-gathered run follows the oracle run with BF16-rounded gathers.
+    # Only the first sweep starts from identical FP32 factors; afterwards FP32
+    # accumulation differences (~1e-7) flip BF16 roundings at ties, each a 2^-9
+    # relative jump, so the trajectories separate by ~1e-4 over 10 sweeps
+    # (the oracle's own BF16 and FP64 runs differ by as much).
+    assert abs(mixed.fit_history[0] - fitsbf[0]) < 1e-5, (mixed.fit_history[0], fitsbf[0])
+    np.testing.assert_allclose(mixed.fit_history, fitsbf, rtol=0, atol=2e-4)
+    # equal fit within the same spread: plain vs mixed on this tensor
+    assert abs(mixed.final_fit - plain.final_fit) < 5e-4, (mixed.final_fit, plain.final_fit)
 
 
 def test_c4_shape_jacobi_true_fit():
