@@ -45,10 +45,39 @@ def log(*a):
 
 
 def algorithmic_bytes(dims, nnz, R, mode, bf=4):
-    """B_n = nnz (4 + 4 (N-1)) + 8 (I_n + 1) + sum_{w != n} I_w R b_f + 4 I_n R."""
+    """Compulsory HBM bytes of one mode-n spMTTKRP:
+    B_n = nnz (4 + 4 (N-1))            values + the other modes' index columns
+        + 4 (I_n + 1)                  row pointer
+        + sum_{w != n} I_w R b_f       every gathered factor row read once
+        + 4 I_n R                      output rows written once."""
     N = len(dims)
     gathered = sum(dims[w] * R * bf for w in range(N) if w != mode)
-    return nnz * (4 + 4 * (N - 1)) + 8 * (dims[mode] + 1) + gathered + 4 * dims[mode] * R
+    return nnz * (4 + 4 * (N - 1)) + 4 * (dims[mode] + 1) + gathered + 4 * dims[mode] * R
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "spmttkrp_c2_hbm_traffic.md"
+
+
+def measured_traffic(workload):
+    """HBM bytes per spMTTKRP launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy), or None."""
+    if workload != "c2" or not TRAFFIC_FILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_FILE.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(mib=1024, reps=5):
+    """A plain device-to-device copy of known size, for calibrating the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) under rocprofv3."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
 
 
 def cpu_baseline(T, wl, threads, iters=3):
@@ -98,11 +127,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the BF16-gather comparison")
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the engine is single-device; multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from dynasor_plus_amd import GpuContext, host, init_factors
     from dynasor_plus_amd import _lib as L
@@ -192,7 +226,8 @@ def main():
         "remap_s": remap_s,
         "final_fit": res.final_fit,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload)},
         "mixed": mixed,
         "cpu_baseline": cpu,
     }

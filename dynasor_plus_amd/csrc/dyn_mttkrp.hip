@@ -13,13 +13,13 @@
 //     of the row, loaded 16 bytes at a time (4 FP32 or 8 BF16 columns per
 //     load, widened to FP32), so one quad reads a whole factor row per
 //     operand and the per-nonzero bookkeeping is paid by 4 lanes only,
-//   * persistent workgroups (one per CU) first drain the heavy-row pieces,
-//     one wave per piece: the wave loads the piece's kPiece values and index
-//     columns with coalesced loads (kPieceLane per lane, all in flight at
-//     once), quad g then takes the 4 nonzeros held by its own lanes in each
-//     64-nonzero round (DPP quad broadcasts), and the 16 quad partial rows are
-//     summed across the wave into the piece's slot.  A second kernel sums
-//     each heavy row's slots in slot order,
+//   * persistent workgroups (one per CU) first process the heavy-row pieces:
+//     every wave owns a contiguous range of pieces; per piece it loads the
+//     piece's values and index columns up front (kPieceLane per lane), quad g
+//     takes nonzeros [16 g, 16 g + 16) in rounds of 4 (DPP quad broadcasts),
+//     and the wave keeps a running row sum that it reduces across its 16
+//     quads and stores once per row it touches.  A second kernel sums each
+//     heavy row's stored partials in slot order,
 //   * then the light segments, one quad each, walked in batches of 4 (next
 //     batch prefetched), rows stored as they complete,
 //   * factor rows are gathered by frequency rank: the hottest rows of the
@@ -54,6 +54,7 @@ struct SegArgs {
     float*          out;
     float*          part;         // [npieces][RP]
     double*         loss_dev;     // completion: summed loss (device scalar)
+    uint32_t        nwaves;       // waves of the launch (set by the launcher)
     // completion kernel only
     const float*    An;           // output-mode factor rows (FP32, input order)
     double*         loss_part;    // [npieces + nseg] or null
@@ -273,6 +274,19 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const ui
     }
 }
 
+// Contiguous split of P pieces over W waves: wave w owns
+// [range_begin(w), range_begin(w + 1)) with P = W q + rem, the first rem
+// waves taking q + 1 pieces.
+__device__ __forceinline__ uint32_t range_begin(uint32_t w, uint32_t q, uint32_t rem) {
+    return w * q + min(w, rem);
+}
+// whether piece p is the last of its wave's range
+__device__ __forceinline__ bool range_last(uint32_t p, uint32_t q, uint32_t rem) {
+    const uint32_t big = rem * (q + 1);
+    const uint32_t w = (p < big) ? p / (q + 1) : rem + (p - big) / q;
+    return p + 1 == range_begin(w + 1, q, rem);
+}
+
 // KIND 0: spMTTKRP / adjoint.  KIND 1: fused completion loss + gradient.
 template <int RP, int NO, typename FT, int KIND>
 __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a) {
@@ -312,59 +326,80 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane(
         (int)(blockIdx.x * (uint32_t)(TB / 64) + (uint32_t)(tid >> 6)));
 
-    // ---- phase 1: heavy-row pieces, one wave each ----
-    for (uint32_t p = gw; p < a.npieces; p += nwaves) {
-        const uint32_t kb = a.pc_k[p];
-        const uint32_t n = a.pc_e[p] - kb;
-        const uint32_t row = a.pc_row[p];
-        // the whole piece's stream: quad g owns nonzeros [16 g, 16 g + 16) of the
-        // piece, lane q of it holds 16 g + 4 t + q for t < kPieceLane
-        float sv[PL];
-        uint32_t si[PL][NO];
-#pragma unroll
-        for (int t = 0; t < PL; ++t) {
-            const uint32_t k = (uint32_t)(16 * qw + 4 * t + q);
-            const bool in = k < n;
-            sv[t] = in ? (a.vperm ? a.vals[a.vperm[kb + k]] : a.vals[kb + k]) : 0.f;
-#pragma unroll
-            for (int w = 0; w < NO; ++w) si[t][w] = in ? a.oidx[w][kb + k] : 0u;
-        }
+    // ---- phase 1: heavy-row pieces, one wave at a time ----
+    // Wave gw owns pieces [range_begin(gw), range_begin(gw + 1)); its running row
+    // sum is flushed into the slot of the last piece of each row it touches,
+    // which is where k_heavy looks for it.
+    {
+        const uint32_t pq = a.npieces / nwaves, prem = a.npieces % nwaves;
+        const uint32_t p0 = range_begin(gw, pq, prem);
+        const uint32_t p1 = range_begin(gw + 1, pq, prem);
         float acc[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
         float arow[KIND == 1 ? CPL : 1];
-        if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
         double lsum = 0.0;
         uint32_t prev0 = kNone;
         float y0prev[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
-#pragma unroll
-        for (int t = 0; t < PL; ++t) {
-            const uint32_t k0 = (uint32_t)(16 * qw + 4 * t);
-            if (k0 < n) {
-                float vb[4];
-                uint32_t ib[4][NO];
-                bcast_batch<0, NO>(sv[t], si[t], vb[0], ib[0]);
-                bcast_batch<1, NO>(sv[t], si[t], vb[1], ib[1]);
-                bcast_batch<2, NO>(sv[t], si[t], vb[2], ib[2]);
-                bcast_batch<3, NO>(sv[t], si[t], vb[3], ib[3]);
-                batch4<RP, NO, FT, KIND, HB, CPL>(
-                    a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
-                    [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
+        for (uint32_t p = p0; p < p1; ++p) {
+            const uint32_t kb = a.pc_k[p];
+            const uint32_t n = a.pc_e[p] - kb;
+            const uint32_t row = a.pc_row[p];
+            if constexpr (KIND == 1) {
+                if (p == p0 || a.pc_row[p - 1] != row)
+                    load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
             }
-        }
-        // sum the 16 quad partials (butterfly over quads, fixed order)
+            // the piece's stream: quad g owns nonzeros [16 g, 16 g + 16), lane q
+            // of it holds 16 g + 4 t + q for t < kPieceLane
+            float sv[PL];
+            uint32_t si[PL][NO];
 #pragma unroll
-        for (int o = 4; o < 64; o <<= 1)
+            for (int t = 0; t < PL; ++t) {
+                const uint32_t k = (uint32_t)(16 * qw + 4 * t + q);
+                const bool in = k < n;
+                sv[t] = in ? (a.vperm ? a.vals[a.vperm[kb + k]] : a.vals[kb + k]) : 0.f;
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
-        if (qw == 0) store_row<RP, FT>(a.part + (size_t)p * RP, q, acc);
-        if constexpr (KIND == 1) {
-            if (a.loss_part) {
+                for (int w = 0; w < NO; ++w) si[t][w] = in ? a.oidx[w][kb + k] : 0u;
+            }
 #pragma unroll
-                for (int o = 4; o < 64; o <<= 1) lsum += __shfl_xor(lsum, o, 64);
-                if (lane == 0) a.loss_part[p] = lsum;
+            for (int t = 0; t < PL; ++t) {
+                const uint32_t k0 = (uint32_t)(16 * qw + 4 * t);
+                if (k0 < n) {
+                    float vb[4];
+                    uint32_t ib[4][NO];
+                    bcast_batch<0, NO>(sv[t], si[t], vb[0], ib[0]);
+                    bcast_batch<1, NO>(sv[t], si[t], vb[1], ib[1]);
+                    bcast_batch<2, NO>(sv[t], si[t], vb[2], ib[2]);
+                    bcast_batch<3, NO>(sv[t], si[t], vb[3], ib[3]);
+                    batch4<RP, NO, FT, KIND, HB, CPL>(
+                        a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
+                        [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
+                }
+            }
+            const bool flush = (p + 1 == p1) || (a.pc_row[p + 1] != row);
+            if (flush) {
+                // sum the 16 quad partials (butterfly over quads, fixed order)
+#pragma unroll
+                for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
+                if (qw == 0) store_row<RP, FT>(a.part + (size_t)p * RP, q, acc);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
+            }
+            if constexpr (KIND == 1) {
+                if (a.loss_part) {
+                    double l = 0.0;
+                    if (flush) {
+                        l = lsum;
+#pragma unroll
+                        for (int o = 4; o < 64; o <<= 1) l += __shfl_xor(l, o, 64);
+                        lsum = 0.0;
+                    }
+                    if (lane == 0) a.loss_part[p] = l;
+                }
             }
         }
     }
@@ -439,13 +474,15 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     }
 }
 
-// Heavy rows: sum the row's piece partials in slot order (quads stride the
-// slots, then an ordered LDS reduction).  One block per heavy row.
+// Heavy rows: sum the row's flushed partials (the row's last piece, and the
+// last piece of every wave range inside the row) in slot order -- quads
+// stride the slots, then an ordered LDS reduction.  One block per heavy row.
 template <int RP>
 __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part,
                                                   const uint32_t* __restrict__ hrow,
                                                   const uint32_t* __restrict__ hfirst,
                                                   const uint32_t* __restrict__ hcount,
+                                                  uint32_t npieces, uint32_t nwaves,
                                                   float* __restrict__ out) {
     using C = Cols<RP, float>;
     constexpr int CPL = C::CPL;
@@ -455,14 +492,18 @@ __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part
     const int q = tid & 3;
     const uint32_t h = blockIdx.x;
     const uint32_t first = hfirst[h], cnt = hcount[h];
+    const uint32_t pq = npieces / nwaves, prem = npieces % nwaves;
     float sum[CPL];
 #pragma unroll
     for (int j = 0; j < CPL; ++j) sum[j] = 0.f;
     for (uint32_t s = g; s < cnt; s += kQuads) {
-        float t[CPL];
-        load_row<RP, float>(part + (size_t)(first + s) * RP, q, t);
+        const uint32_t p = first + s;
+        if (s + 1 == cnt || range_last(p, pq, prem)) {
+            float t[CPL];
+            load_row<RP, float>(part + (size_t)p * RP, q, t);
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) sum[j] += t[j];
+            for (int j = 0; j < CPL; ++j) sum[j] += t[j];
+        }
     }
     store_row<RP, float>(&red[g][0], q, sum);
     __syncthreads();
@@ -501,6 +542,7 @@ static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
                                   (int)std::min<size_t>(c->max_lds, 160u * 1024u));
         attr_set = true;
     }
+    a.nwaves = (uint32_t)c->num_cus * (uint32_t)(seg_threads<RP, NO, KIND>() / 64);
     hipLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()), bytes, c->stream, a);
     return DYN_OK;
 }
@@ -531,15 +573,19 @@ static int dispatch_rp(dyn_gpu_ctx* c, int Rp, int NO, SegArgs& a, const uint32_
 }
 
 static int launch_heavy(dyn_gpu_ctx* c, int Rp, const ModeCopy& cp, const float* part,
-                        float* out) {
+                        uint32_t nwaves, float* out) {
     if (cp.nheavy == 0) return DYN_OK;
+#define DYN_HEAVY(RPV)                                                                        \
+    hipLaunchKernelGGL(k_heavy<RPV>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part,     \
+                       cp.heavy_row, cp.heavy_first, cp.heavy_count, cp.npieces, nwaves, out)
     switch (Rp) {
-        case 16: hipLaunchKernelGGL(k_heavy<16>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
-        case 32: hipLaunchKernelGGL(k_heavy<32>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
-        case 64: hipLaunchKernelGGL(k_heavy<64>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
-        case 128: hipLaunchKernelGGL(k_heavy<128>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first, cp.heavy_count, out); break;
+        case 16: DYN_HEAVY(16); break;
+        case 32: DYN_HEAVY(32); break;
+        case 64: DYN_HEAVY(64); break;
+        case 128: DYN_HEAVY(128); break;
         default: return set_err(c, DYN_ERR_UNSUPPORTED, "unsupported padded rank %d", Rp);
     }
+#undef DYN_HEAVY
     return check_launch(c, "heavy_rows");
 }
 
@@ -600,7 +646,7 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
         DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a, dims_o)));
     DYN_CHECK(check_launch(c, "mttkrp_seg"));
     timing_end(c, &tp);
-    return launch_heavy(c, Rp, c->copy[mode], a.part, out);
+    return launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, out);
 }
 
 // Relabelled copies (rows in frequency-rank order) of caller factors A[n]
@@ -803,7 +849,7 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         if (a.nseg == 0 && a.npieces == 0) continue;
         DYN_CHECK((dispatch_rp<float, 1>(c, Rp, NO, a, dims_o)));
         DYN_CHECK(check_launch(c, "completion_seg"));
-        DYN_CHECK(launch_heavy(c, Rp, c->copy[mode], a.part, grad[mode]));
+        DYN_CHECK(launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, grad[mode]));
         if (want_loss) {
             double* d = a.loss_dev;
             hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, a.loss_part,

@@ -224,10 +224,25 @@ __global__ void k_gather_relabel(const uint32_t* __restrict__ src, const uint32_
     for (; i < n; i += stride) dst[i] = relab[src[perm[i]]];
 }
 
-__global__ void k_bincount(const uint32_t* __restrict__ ix, uint32_t n, uint32_t* __restrict__ cnt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (; i < n; i += stride) atomicAdd(&cnt[ix[i]], 1u);
+// Histogram of ix restricted to bins [lo, lo + nb): per-workgroup LDS counts
+// (contention on hot bins stays inside the CU), then one global add per
+// non-empty bin and workgroup.
+constexpr int kHistBlock = 1024;
+constexpr uint32_t kHistBins = 32768;
+
+__global__ __launch_bounds__(kHistBlock) void k_bincount(const uint32_t* __restrict__ ix,
+                                                         uint32_t n, uint32_t lo, uint32_t nb,
+                                                         uint32_t* __restrict__ cnt) {
+    extern __shared__ uint32_t hist[];
+    for (uint32_t b = threadIdx.x; b < nb; b += kHistBlock) hist[b] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kHistBlock + threadIdx.x; i < n; i += gridDim.x * kHistBlock) {
+        const uint32_t v = ix[i] - lo;
+        if (v < nb) atomicAdd(&hist[v], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += kHistBlock)
+        if (hist[b]) atomicAdd(&cnt[lo + b], hist[b]);
 }
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
@@ -297,10 +312,23 @@ int ensure_relabel(dyn_gpu_ctx* c) {
         uint32_t* cnt = (uint32_t*)sp;
         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(dim)), dim3(kBlock), 0, c->stream, cnt,
                            (size_t)dim);
-        if (c->nnz)
-            hipLaunchKernelGGL(k_bincount, dim3(grid_for(c->nnz)), dim3(kBlock), 0, c->stream,
-                               c->idx[w], (uint32_t)c->nnz, cnt);
-        DYN_CHECK(check_launch(c, "bincount"));
+        if (c->nnz) {
+            const uint32_t nblk = (uint32_t)std::min<uint64_t>(
+                std::max<uint64_t>((c->nnz + 65535) / 65536, 1), (uint64_t)c->num_cus * 4);
+            static bool attr_set = false;
+            if (!attr_set) {
+                (void)hipFuncSetAttribute((const void*)k_bincount,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(kHistBins * sizeof(uint32_t)));
+                attr_set = true;
+            }
+            for (uint32_t lo = 0; lo < dim; lo += kHistBins) {
+                const uint32_t nb = std::min(kHistBins, dim - lo);
+                hipLaunchKernelGGL(k_bincount, dim3(nblk), dim3(kHistBlock), nb * sizeof(uint32_t),
+                                   c->stream, c->idx[w], (uint32_t)c->nnz, lo, nb, cnt);
+                DYN_CHECK(check_launch(c, "bincount"));
+            }
+        }
         std::vector<uint32_t> h(dim);
         DYN_HIP(c, hipMemcpyAsync(h.data(), cnt, (size_t)dim * 4, hipMemcpyDeviceToHost, c->stream));
         DYN_HIP(c, hipStreamSynchronize(c->stream));
