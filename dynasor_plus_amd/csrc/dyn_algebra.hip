@@ -568,9 +568,9 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
 //  host wrappers
 // ---------------------------------------------------------------------------
 static uint32_t gram_wgs(uint32_t rows, int RP) {
-    // ~1024 rows per workgroup: enough workgroups to stream large factors,
-    // few enough partial Grams for the reduction
-    const uint32_t per = (RP <= 64) ? 1024u : 512u;
+    // ~128 rows per workgroup: a few hundred workgroups for the largest
+    // factors (the pass is latency-bound), and the partial Grams stay small
+    const uint32_t per = 128u;
     uint32_t want = (rows + per - 1) / per;
     return std::max<uint32_t>(1, std::min<uint32_t>(want, kMaxGramWG));
 }

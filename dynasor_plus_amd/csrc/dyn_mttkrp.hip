@@ -280,11 +280,10 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const ui
 __device__ __forceinline__ uint32_t range_begin(uint32_t w, uint32_t q, uint32_t rem) {
     return w * q + min(w, rem);
 }
-// whether piece p is the last of its wave's range
-__device__ __forceinline__ bool range_last(uint32_t p, uint32_t q, uint32_t rem) {
+// the wave whose range holds piece p
+__device__ __forceinline__ uint32_t wave_of(uint32_t p, uint32_t q, uint32_t rem) {
     const uint32_t big = rem * (q + 1);
-    const uint32_t w = (p < big) ? p / (q + 1) : rem + (p - big) / q;
-    return p + 1 == range_begin(w + 1, q, rem);
+    return (p < big) ? p / (q + 1) : rem + (p - big) / q;
 }
 
 // KIND 0: spMTTKRP / adjoint.  KIND 1: fused completion loss + gradient.
@@ -343,13 +342,14 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
         float y0prev[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
+        bool whole = false;   // the current row starts inside this range
         for (uint32_t p = p0; p < p1; ++p) {
             const uint32_t kb = a.pc_k[p];
             const uint32_t n = a.pc_e[p] - kb;
             const uint32_t row = a.pc_row[p];
-            if constexpr (KIND == 1) {
-                if (p == p0 || a.pc_row[p - 1] != row)
-                    load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
+            if (p == p0 || a.pc_row[p - 1] != row) {   // the row's first piece in this range
+                whole = (p > p0) || (p == 0) || (a.pc_row[p - 1] != row);
+                if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
             }
             // the piece's stream: quad g owns nonzeros [16 g, 16 g + 16), lane q
             // of it holds 16 g + 4 t + q for t < kPieceLane
@@ -378,14 +378,17 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                         [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
                 }
             }
-            const bool flush = (p + 1 == p1) || (a.pc_row[p + 1] != row);
+            const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
+            const bool flush = row_ends || (p + 1 == p1);
             if (flush) {
-                // sum the 16 quad partials (butterfly over quads, fixed order)
+                // sum the 16 quad partials (butterfly over quads, fixed order);
+                // a row held whole by this wave is final, else a partial
 #pragma unroll
                 for (int o = 4; o < 64; o <<= 1)
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
-                if (qw == 0) store_row<RP, FT>(a.part + (size_t)p * RP, q, acc);
+                float* dst = (whole && row_ends) ? a.out + (size_t)row * RP : a.part + (size_t)p * RP;
+                if (qw == 0) store_row<RP, FT>(dst, q, acc);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) acc[c] = 0.f;
             }
@@ -474,51 +477,44 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     }
 }
 
-// Heavy rows: sum the row's flushed partials (the row's last piece, and the
-// last piece of every wave range inside the row) in slot order -- quads
-// stride the slots, then an ordered LDS reduction.  One block per heavy row.
+// Heavy rows split over several waves' piece ranges: sum the partials the
+// waves stored (the last piece of each range inside the row, and the row's
+// last piece) in slot order -- quads stride the ranges, then a butterfly
+// over quads.  One wave per heavy row; rows held whole by one wave were
+// stored by the segment kernel and are skipped.
 template <int RP>
 __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part,
                                                   const uint32_t* __restrict__ hrow,
                                                   const uint32_t* __restrict__ hfirst,
                                                   const uint32_t* __restrict__ hcount,
-                                                  uint32_t npieces, uint32_t nwaves,
-                                                  float* __restrict__ out) {
+                                                  uint32_t nheavy, uint32_t npieces,
+                                                  uint32_t nwaves, float* __restrict__ out) {
     using C = Cols<RP, float>;
     constexpr int CPL = C::CPL;
-    __shared__ float red[kQuads][RP];
-    const int tid = threadIdx.x;
-    const int g = tid >> 2;
-    const int q = tid & 3;
-    const uint32_t h = blockIdx.x;
-    const uint32_t first = hfirst[h], cnt = hcount[h];
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 2;
+    const int q = lane & 3;
+    const uint32_t h = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (h >= nheavy) return;
+    const uint32_t first = hfirst[h], last = first + hcount[h] - 1;
     const uint32_t pq = npieces / nwaves, prem = npieces % nwaves;
+    const uint32_t w0 = wave_of(first, pq, prem), w1 = wave_of(last, pq, prem);
+    if (w0 == w1) return;
     float sum[CPL];
 #pragma unroll
     for (int j = 0; j < CPL; ++j) sum[j] = 0.f;
-    for (uint32_t s = g; s < cnt; s += kQuads) {
-        const uint32_t p = first + s;
-        if (s + 1 == cnt || range_last(p, pq, prem)) {
-            float t[CPL];
-            load_row<RP, float>(part + (size_t)p * RP, q, t);
-#pragma unroll
-            for (int j = 0; j < CPL; ++j) sum[j] += t[j];
-        }
-    }
-    store_row<RP, float>(&red[g][0], q, sum);
-    __syncthreads();
-    if (g == 0) {
+    for (uint32_t w = w0 + (uint32_t)g; w <= w1; w += 16) {
+        const uint32_t slot = min(range_begin(w + 1, pq, prem) - 1, last);
         float t[CPL];
-        load_row<RP, float>(&red[0][0], q, t);
-        const int used = (int)min(cnt, (uint32_t)kQuads);
-        for (int j = 1; j < used; ++j) {
-            float u[CPL];
-            load_row<RP, float>(&red[j][0], q, u);
+        load_row<RP, float>(part + (size_t)slot * RP, q, t);
 #pragma unroll
-            for (int k = 0; k < CPL; ++k) t[k] += u[k];
-        }
-        store_row<RP, float>(out + (size_t)hrow[h] * RP, q, t);
+        for (int j = 0; j < CPL; ++j) sum[j] += t[j];
     }
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) sum[j] += __shfl_xor(sum[j], o, 64);
+    if (g == 0) store_row<RP, float>(out + (size_t)hrow[h] * RP, q, sum);
 }
 
 // ---------------------------------------------------------------------------
@@ -576,8 +572,9 @@ static int launch_heavy(dyn_gpu_ctx* c, int Rp, const ModeCopy& cp, const float*
                         uint32_t nwaves, float* out) {
     if (cp.nheavy == 0) return DYN_OK;
 #define DYN_HEAVY(RPV)                                                                        \
-    hipLaunchKernelGGL(k_heavy<RPV>, dim3(cp.nheavy), dim3(kBlock), 0, c->stream, part,     \
-                       cp.heavy_row, cp.heavy_first, cp.heavy_count, cp.npieces, nwaves, out)
+    hipLaunchKernelGGL(k_heavy<RPV>, dim3((cp.nheavy + kBlock / 64 - 1) / (kBlock / 64)),  \
+                       dim3(kBlock), 0, c->stream, part, cp.heavy_row, cp.heavy_first,       \
+                       cp.heavy_count, cp.nheavy, cp.npieces, nwaves, out)
     switch (Rp) {
         case 16: DYN_HEAVY(16); break;
         case 32: DYN_HEAVY(32); break;
