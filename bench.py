@@ -45,14 +45,15 @@ def log(*a):
 
 
 def algorithmic_bytes(dims, nnz, R, mode, bf=4):
-    """Compulsory HBM bytes of one mode-n spMTTKRP:
+    """Compulsory HBM bytes of one mode-n spMTTKRP (the reference's CSR-compact
+    accounting, src/dynasor_kernel.cpp:585-595):
     B_n = nnz (4 + 4 (N-1))            values + the other modes' index columns
-        + 4 (I_n + 1)                  row pointer
+        + 8 (I_n + 1)                  row pointer / segment table
         + sum_{w != n} I_w R b_f       every gathered factor row read once
         + 4 I_n R                      output rows written once."""
     N = len(dims)
     gathered = sum(dims[w] * R * bf for w in range(N) if w != mode)
-    return nnz * (4 + 4 * (N - 1)) + 4 * (dims[mode] + 1) + gathered + 4 * dims[mode] * R
+    return nnz * (4 + 4 * (N - 1)) + 8 * (dims[mode] + 1) + gathered + 4 * dims[mode] * R
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "spmttkrp_c2_hbm_traffic.md"
