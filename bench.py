@@ -37,6 +37,16 @@ WORKLOADS = {
     "small": dict(dims=(1200, 900, 2800), nnz=2_000_000, coords="zipf", alpha=1.1,
                   values="lognormal", rank=32, seed=2,
                   desc="3-mode 1200x900x2800, 2M distinct Zipf(1.1) coords, rank 32"),
+    # parity-test configurations, runnable by name (not the default line)
+    "c3": dict(dims=(1_000_000, 200_000, 20_000, 500), nnz=150_000_000, coords="zipf", alpha=1.2,
+               values="lognormal", rank=64, seed=3,
+               desc="4-mode 1Mx200kx20kx500, 150M distinct Zipf(1.2) coords, lognormal(0,1) "
+                    "values, CP-ALS Gauss-Seidel rank 64"),
+    "c4": dict(dims=(500_000, 100_000, 50_000, 5_000, 100), nnz=400_000_000, coords="zipf",
+               alpha=1.0, values="planted", planted_rank=20, noise_frac=0.05, rank=32, seed=4,
+               kernel="jacobi",
+               desc="5-mode 500kx100kx50kx5kx100, 400M distinct Zipf(1.0) coords, planted rank 20 "
+                    "+ 5% noise, CP-ALS Jacobi rank 32"),
 }
 
 
@@ -145,8 +155,9 @@ def main():
     wl = WORKLOADS[args.workload]
     threads = args.threads or min(16, os.cpu_count() or 1)
     t0 = time.time()
+    gen_extra = {k: wl[k] for k in ("planted_rank", "noise_frac") if k in wl}
     T, _, st = host.generate(wl["dims"], wl["nnz"], seed=wl["seed"], coords=wl["coords"],
-                             alpha=wl["alpha"], values=wl["values"], threads=threads)
+                             alpha=wl["alpha"], values=wl["values"], threads=threads, **gen_extra)
     log(f"generated {T.nnz} nnz in {time.time() - t0:.1f}s (rounds={st.rounds}, "
         f"max row={st.max_row_nnz_mode0})")
     R = wl["rank"]
@@ -156,17 +167,18 @@ def main():
     t0 = time.time()
     ctx.remap_all()
     remap_s = time.time() - t0
-    log(f"remap (3 per-mode copies) {remap_s:.3f}s")
+    log(f"remap ({len(dims)} per-mode copies) {remap_s:.3f}s")
     Y0 = init_factors(dims, R, 42, "normal")
+    kern = L.DYN_CP_JACOBI if wl.get("kernel") == "jacobi" else L.DYN_CP_GAUSS_SEIDEL
 
     # ---- plain FP32 gathers ----
     ctx.set_factors(Y0, R)
-    ctx.run_cpals(args.warmup, 0.0)
+    ctx.run_cpals(args.warmup, 0.0, kernel=kern)
     ctx.enable_timing(True)
     ctx.kernel_stats()                      # reset
     ctx.synchronize()
     t0 = time.perf_counter()
-    res = ctx.run_cpals(args.steps, 0.0)
+    res = ctx.run_cpals(args.steps, 0.0, kernel=kern)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     launches, kms = ctx.kernel_stats()
@@ -184,11 +196,11 @@ def main():
     if not args.no_mixed:
         iters = args.warmup + args.steps
         ctx.set_factors(Y0, R)
-        rp = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_FP32)
+        rp = ctx.run_cpals(iters, 0.0, kernel=kern, precision=L.DYN_PREC_FP32)
         ctx.set_factors(Y0, R)
         ctx.enable_timing(True)
         ctx.kernel_stats()
-        rm = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_BF16)
+        rm = ctx.run_cpals(iters, 0.0, kernel=kern, precision=L.DYN_PREC_BF16)
         ml, mms = ctx.kernel_stats()
         ctx.enable_timing(False)
         med = lambda h: statistics.median(h[1:]) if len(h) > 1 else h[0]
@@ -202,7 +214,7 @@ def main():
     ctx.close()
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload in ("c2", "small"):
         t0 = time.time()
         cpu = cpu_baseline(T, wl, threads)
         log(f"cpu baseline {cpu} ({time.time() - t0:.1f}s)")
@@ -221,7 +233,7 @@ def main():
         "dtype": "f32 (gathers/accumulate), f64 (Cholesky, lambda, fit)",
         "data": "synthetic (engine generator, distinct coordinates)",
         "config": {"workload": wl["desc"], "dims": list(dims), "nnz": T.nnz, "rank": R,
-                   "kernel": "gauss-seidel", "parallelism": "single GPU"},
+                   "kernel": wl.get("kernel", "gauss-seidel"), "parallelism": "single GPU"},
         "spmttkrp_GBps": achieved,
         "spmttkrp_ms_per_mode": kms / launches if launches else None,
         "remap_s": remap_s,

@@ -72,6 +72,7 @@ struct Cols {
     static constexpr int LW = (sizeof(FT) == 2 && RP >= 32) ? 8 : 4;
     static constexpr int NB = RP / (4 * LW);
     static constexpr int CPL = RP / 4;         // columns per lane
+    static constexpr int RD = RP * (int)sizeof(FT) / 16;   // raw 32-bit words per lane
     static __device__ __forceinline__ int col(int b, int q, int t) { return b * 4 * LW + q * LW + t; }
 };
 
@@ -119,6 +120,38 @@ __device__ __forceinline__ void store_row(float* __restrict__ row, int q, const 
             *reinterpret_cast<float4*>(row + C::col(b, q, h * 4)) =
                 make_float4(v[t], v[t + 1], v[t + 2], v[t + 3]);
         }
+}
+
+// A lane's part of a row as raw 32-bit words (no conversion), and its
+// conversion to FP32 columns in load_row's order.  Gathers load raw words so
+// that a conditional gather needs no wait inside its branch.
+template <int RP, typename FT>
+__device__ __forceinline__ void load_raw(const FT* __restrict__ row, int q, uint32_t* r) {
+    using C = Cols<RP, FT>;
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b) {
+        if constexpr (C::LW * sizeof(FT) == 16) {
+            const uint4 x = *reinterpret_cast<const uint4*>(row + C::col(b, q, 0));
+            r[b * 4 + 0] = x.x;
+            r[b * 4 + 1] = x.y;
+            r[b * 4 + 2] = x.z;
+            r[b * 4 + 3] = x.w;
+        } else {
+            const uint2 x = *reinterpret_cast<const uint2*>(row + C::col(b, q, 0));
+            r[b * 2 + 0] = x.x;
+            r[b * 2 + 1] = x.y;
+        }
+    }
+}
+
+template <int RP, typename FT>
+__device__ __forceinline__ void widen_raw(const uint32_t* r, float* v) {
+    using C = Cols<RP, FT>;
+#pragma unroll
+    for (int i = 0; i < C::RD; ++i) {
+        if constexpr (sizeof(FT) == 4) v[i] = __uint_as_float(r[i]);
+        else widen(r[i], v[2 * i], v[2 * i + 1]);
+    }
 }
 
 // quad broadcast of lane J (DPP quad_perm [J,J,J,J])
@@ -171,6 +204,17 @@ constexpr int seg_hoist() {
     return seg_regs<RP, NO, KIND>(4) <= budget ? 4 : (seg_regs<RP, NO, KIND>(2) <= budget ? 2 : 1);
 }
 
+// Heavy pieces: rounds of 4 nonzeros per quad whose gathers are in flight
+// together -- two when the raw words of 8 nonzeros fit beside the rest.
+template <int RP, int NO, typename FT, int KIND>
+constexpr int seg_rounds() {
+    return (KIND == 0 && seg_hoist<RP, NO, KIND>() == 4 &&
+            seg_threads<RP, NO, KIND>() == 1024 &&
+            8 * NO * Cols<RP, FT>::RD + seg_regs<RP, NO, KIND>(4) - 4 * NO * (RP / 4) <= 128)
+               ? 2
+               : 1;
+}
+
 template <int J, int NO>
 __device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, float& v, uint32_t* iw) {
     v = qbf<J>(v_l);
@@ -178,12 +222,12 @@ __device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, floa
     for (int w = 0; w < NO; ++w) iw[w] = qb<J>(i_l[w]);
 }
 
-// a factor row by frequency rank: from LDS when hot, else from memory
+// a factor row by frequency rank (raw words): from LDS when hot, else from memory
 template <int RP, typename FT>
-__device__ __forceinline__ void gather_row(const FT* __restrict__ g, const FT* lds_tab,
-                                           uint32_t hot, uint32_t i, int q, float* v) {
-    if (i < hot) load_row<RP, FT>(lds_tab + (size_t)i * RP, q, v);
-    else load_row<RP, FT>(g + (size_t)i * RP, q, v);
+__device__ __forceinline__ void gather_raw(const FT* __restrict__ g, const FT* lds_tab,
+                                           uint32_t hot, uint32_t i, int q, uint32_t* r) {
+    if (i < hot) load_raw<RP, FT>(lds_tab + (size_t)i * RP, q, r);
+    else load_raw<RP, FT>(g + (size_t)i * RP, q, r);
 }
 
 // One nonzero's contribution to the quad's accumulator (y[w] = gathered rows).
@@ -228,19 +272,21 @@ __device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)
     }
 }
 
-// Gathers of HB broadcast nonzeros (operand 0 reused from y0prev when the
+// Gathers of HB broadcast nonzeros (operand 0 reused from raw0prev when the
 // index repeats), then their accumulation in order.  `live(j)` says whether
 // nonzero j of the batch exists; `before_step(j)` runs ahead of its
 // accumulation (row changes of light segments).
-template <int RP, int NO, typename FT, int KIND, int HB, int CPL, typename Live, typename Before>
+template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, typename Live,
+          typename Before>
 __device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const uint32_t* off,
                                        const float* vb, const uint32_t (*ib)[NO], int q,
                                        float* acc, const float* arow, double* lsum,
-                                       uint32_t& prev0, float* y0prev, Live live,
+                                       uint32_t& prev0, uint32_t* raw0prev, Live live,
                                        Before before_step) {
+    constexpr int RD = Cols<RP, FT>::RD;
 #pragma unroll
-    for (int j0 = 0; j0 < 4; j0 += HB) {
-        float y[HB][NO][CPL];
+    for (int j0 = 0; j0 < J; j0 += HB) {
+        uint32_t raw[HB][NO][RD];
         bool reuse[HB];
 #pragma unroll
         for (int h = 0; h < HB; ++h) {
@@ -248,26 +294,26 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const ui
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
             reuse[h] = (ib[j][0] == before);
             if (!reuse[h])
-                gather_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[0]), lds + (size_t)off[0] * RP,
-                                   a.hot[0], ib[j][0], q, y[h][0]);
+                gather_raw<RP, FT>(reinterpret_cast<const FT*>(a.Yo[0]), lds + (size_t)off[0] * RP,
+                                   a.hot[0], ib[j][0], q, raw[h][0]);
 #pragma unroll
             for (int w = 1; w < NO; ++w)
-                gather_row<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]), lds + (size_t)off[w] * RP,
-                                   a.hot[w], ib[j][w], q, y[h][w]);
+                gather_raw<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]), lds + (size_t)off[w] * RP,
+                                   a.hot[w], ib[j][w], q, raw[h][w]);
         }
 #pragma unroll
         for (int h = 0; h < HB; ++h) {
             const int j = j0 + h;
             if (live(j)) {
-                if (reuse[h]) {
 #pragma unroll
-                    for (int c = 0; c < CPL; ++c) y[h][0][c] = y0prev[c];
-                }
+                for (int d = 0; d < RD; ++d) raw0prev[d] = reuse[h] ? raw0prev[d] : raw[h][0][d];
+                float y[NO][CPL];
+                widen_raw<RP, FT>(raw0prev, y[0]);
+#pragma unroll
+                for (int w = 1; w < NO; ++w) widen_raw<RP, FT>(raw[h][w], y[w]);
                 before_step(j);
-                accumulate<CPL, NO, KIND>(acc, vb[j], y[h], arow, a.loss, lsum, q,
+                accumulate<CPL, NO, KIND>(acc, vb[j], y, arow, a.loss, lsum, q,
                                           a.loss_part != nullptr);
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) y0prev[c] = y[h][0][c];
                 prev0 = ib[j][0];
             }
         }
@@ -294,6 +340,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     constexpr int HB = seg_hoist<RP, NO, KIND>();
     constexpr int TB = seg_threads<RP, NO, KIND>();
     constexpr int PL = (int)kPieceLane;
+    constexpr int NR = seg_rounds<RP, NO, FT, KIND>();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     FT* lds = reinterpret_cast<FT*>(lds_raw);
 
@@ -339,9 +386,9 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
         float arow[KIND == 1 ? CPL : 1];
         double lsum = 0.0;
         uint32_t prev0 = kNone;
-        float y0prev[CPL];
+        uint32_t raw0prev[C::RD];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
+        for (int d = 0; d < C::RD; ++d) raw0prev[d] = 0u;
         bool whole = false;   // the current row starts inside this range
         for (uint32_t p = p0; p < p1; ++p) {
             const uint32_t kb = a.pc_k[p];
@@ -364,17 +411,20 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 for (int w = 0; w < NO; ++w) si[t][w] = in ? a.oidx[w][kb + k] : 0u;
             }
 #pragma unroll
-            for (int t = 0; t < PL; ++t) {
+            for (int t = 0; t < PL; t += NR) {
                 const uint32_t k0 = (uint32_t)(16 * qw + 4 * t);
                 if (k0 < n) {
-                    float vb[4];
-                    uint32_t ib[4][NO];
-                    bcast_batch<0, NO>(sv[t], si[t], vb[0], ib[0]);
-                    bcast_batch<1, NO>(sv[t], si[t], vb[1], ib[1]);
-                    bcast_batch<2, NO>(sv[t], si[t], vb[2], ib[2]);
-                    bcast_batch<3, NO>(sv[t], si[t], vb[3], ib[3]);
-                    batch4<RP, NO, FT, KIND, HB, CPL>(
-                        a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
+                    float vb[4 * NR];
+                    uint32_t ib[4 * NR][NO];
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        bcast_batch<0, NO>(sv[t + r], si[t + r], vb[4 * r + 0], ib[4 * r + 0]);
+                        bcast_batch<1, NO>(sv[t + r], si[t + r], vb[4 * r + 1], ib[4 * r + 1]);
+                        bcast_batch<2, NO>(sv[t + r], si[t + r], vb[4 * r + 2], ib[4 * r + 2]);
+                        bcast_batch<3, NO>(sv[t + r], si[t + r], vb[4 * r + 3], ib[4 * r + 3]);
+                    }
+                    batch4<RP, NO, FT, KIND, 4 * NR, (NR > 1 ? 4 * NR : HB), CPL>(
+                        a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
                         [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
                 }
             }
@@ -423,9 +473,9 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
         if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
         double lsum = 0.0;
         uint32_t prev0 = kNone;
-        float y0prev[CPL];
+        uint32_t raw0prev[C::RD];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) y0prev[c] = 0.f;
+        for (int d = 0; d < C::RD; ++d) raw0prev[d] = 0u;
 
         auto fetch = [&](uint32_t kx, float& v_l, uint32_t* i_l) {
             const uint32_t kk = kx + q;
@@ -454,8 +504,8 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
             bcast_batch<1, NO>(v_l, i_l, vb[1], ib[1]);
             bcast_batch<2, NO>(v_l, i_l, vb[2], ib[2]);
             bcast_batch<3, NO>(v_l, i_l, vb[3], ib[3]);
-            batch4<RP, NO, FT, KIND, HB, CPL>(
-                a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, y0prev,
+            batch4<RP, NO, FT, KIND, 4, HB, CPL>(
+                a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
                 [&](int j) { return k0 + (uint32_t)j < ke; },
                 [&](int j) {
                     const uint32_t k = k0 + (uint32_t)j;
