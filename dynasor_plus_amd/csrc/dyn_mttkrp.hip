@@ -126,18 +126,18 @@ __device__ __forceinline__ void store_row(float* __restrict__ row, int q, const 
 // conversion to FP32 columns in load_row's order.  Gathers load raw words so
 // that a conditional gather needs no wait inside its branch.
 template <int RP, typename FT>
-__device__ __forceinline__ void load_raw(const FT* __restrict__ row, int q, uint32_t* r) {
+__device__ __forceinline__ void load_raw(const FT* __restrict__ lane_row, uint32_t* r) {
     using C = Cols<RP, FT>;
 #pragma unroll
     for (int b = 0; b < C::NB; ++b) {
         if constexpr (C::LW * sizeof(FT) == 16) {
-            const uint4 x = *reinterpret_cast<const uint4*>(row + C::col(b, q, 0));
+            const uint4 x = *reinterpret_cast<const uint4*>(lane_row + b * 4 * C::LW);
             r[b * 4 + 0] = x.x;
             r[b * 4 + 1] = x.y;
             r[b * 4 + 2] = x.z;
             r[b * 4 + 3] = x.w;
         } else {
-            const uint2 x = *reinterpret_cast<const uint2*>(row + C::col(b, q, 0));
+            const uint2 x = *reinterpret_cast<const uint2*>(lane_row + b * 4 * C::LW);
             r[b * 2 + 0] = x.x;
             r[b * 2 + 1] = x.y;
         }
@@ -222,13 +222,21 @@ __device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, floa
     for (int w = 0; w < NO; ++w) iw[w] = qb<J>(i_l[w]);
 }
 
-// a factor row by frequency rank (raw words): from LDS when hot, else from memory
+// A factor row by frequency rank (raw words of this lane): from the LDS copy
+// when hot, else from memory.  `gq` / `lq` are the operand's base pointers
+// already offset to this lane's first column.
 template <int RP, typename FT>
-__device__ __forceinline__ void gather_raw(const FT* __restrict__ g, const FT* lds_tab,
-                                           uint32_t hot, uint32_t i, int q, uint32_t* r) {
-    if (i < hot) load_raw<RP, FT>(lds_tab + (size_t)i * RP, q, r);
-    else load_raw<RP, FT>(g + (size_t)i * RP, q, r);
+__device__ __forceinline__ void gather_raw(const FT* gq, const FT* lq, uint32_t hot, uint32_t i,
+                                           uint32_t* r) {
+    load_raw<RP, FT>(((i < hot) ? lq : gq) + (size_t)i * RP, r);
 }
+
+// per-operand gather bases of one lane
+template <int NO, typename FT>
+struct GatherBase {
+    const FT* g[NO];
+    const FT* l[NO];
+};
 
 // One nonzero's contribution to the quad's accumulator (y[w] = gathered rows).
 // KIND 0 forms v*y0, times the middle operands, then a fused multiply-add
@@ -278,7 +286,7 @@ __device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)
 // accumulation (row changes of light segments).
 template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, typename Live,
           typename Before>
-__device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const uint32_t* off,
+__device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT>& gb,
                                        const float* vb, const uint32_t (*ib)[NO], int q,
                                        float* acc, const float* arow, double* lsum,
                                        uint32_t& prev0, uint32_t* raw0prev, Live live,
@@ -293,13 +301,10 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const FT* lds, const ui
             const int j = j0 + h;
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
             reuse[h] = (ib[j][0] == before);
-            if (!reuse[h])
-                gather_raw<RP, FT>(reinterpret_cast<const FT*>(a.Yo[0]), lds + (size_t)off[0] * RP,
-                                   a.hot[0], ib[j][0], q, raw[h][0]);
+            if (!reuse[h]) gather_raw<RP, FT>(gb.g[0], gb.l[0], a.hot[0], ib[j][0], raw[h][0]);
 #pragma unroll
             for (int w = 1; w < NO; ++w)
-                gather_raw<RP, FT>(reinterpret_cast<const FT*>(a.Yo[w]), lds + (size_t)off[w] * RP,
-                                   a.hot[w], ib[j][w], q, raw[h][w]);
+                gather_raw<RP, FT>(gb.g[w], gb.l[w], a.hot[w], ib[j][w], raw[h][w]);
         }
 #pragma unroll
         for (int h = 0; h < HB; ++h) {
@@ -364,6 +369,12 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
         }
     }
     __syncthreads();
+    GatherBase<NO, FT> gb;
+#pragma unroll
+    for (int w = 0; w < NO; ++w) {
+        gb.g[w] = reinterpret_cast<const FT*>(a.Yo[w]) + C::col(0, q, 0);
+        gb.l[w] = lds + (size_t)off[w] * RP + C::col(0, q, 0);
+    }
 
     // Static schedule (pieces have equal length but the last of each row, so
     // round-robin balances them; a shared work counter would serialise on one
@@ -424,7 +435,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                         bcast_batch<3, NO>(sv[t + r], si[t + r], vb[4 * r + 3], ib[4 * r + 3]);
                     }
                     batch4<RP, NO, FT, KIND, 4 * NR, (NR > 1 ? 4 * NR : HB), CPL>(
-                        a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
+                        a, gb, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
                         [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
                 }
             }
@@ -505,7 +516,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
             bcast_batch<2, NO>(v_l, i_l, vb[2], ib[2]);
             bcast_batch<3, NO>(v_l, i_l, vb[3], ib[3]);
             batch4<RP, NO, FT, KIND, 4, HB, CPL>(
-                a, lds, off, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
+                a, gb, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
                 [&](int j) { return k0 + (uint32_t)j < ke; },
                 [&](int j) {
                     const uint32_t k = k0 + (uint32_t)j;

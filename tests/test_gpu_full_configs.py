@@ -1,0 +1,150 @@
+"""The parity-test configurations at their full sizes, through properties that
+do not need an oracle run over the whole tensor:
+
+  * configuration 3 (4-mode 1M x 200k x 20k x 500, 150M Zipf(1.2) coordinates,
+    rank 64): BF16-gathered CP-ALS reaches the plain FP32 run's fit within
+    1e-4 at every iteration from the same initial factors, and the spMTTKRP
+    column invariant holds for both precisions;
+  * configuration 4 (5-mode 500k x 100k x 50k x 5k x 100, 400M Zipf(1.0)
+    coordinates, planted rank 20 + 5 % noise, rank 32): row pointers equal the
+    index bincounts in every mode, the column invariant over all five modes,
+    and one Jacobi and one Gauss-Seidel sweep run to finite fits and lambdas
+    (Jacobi diverges on this tensor within a few sweeps, a property of the
+    update order, so only its first sweep is checked);
+  * configuration 5 (1M x 100k x 2k, 200M uniform coordinates, Bernoulli
+    labels from a planted rank-16 N(0, 0.5) model): 100 Adam steps at rank 32
+    on the fused loss/adjoint, the final device loss against the FP64 oracle
+    loss, and device gradient entries against FP64 central differences of the
+    loss restricted to the nonzeros that the entry touches (the other terms
+    do not depend on it).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from dynasor_plus_amd import GpuContext, host, init_factors
+from dynasor_plus_amd import _lib as L
+from tests._oracle import load as load_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def column_products(ctx, Y, N):
+    """sum_i Yhat_n[i, r] Y_n[i, r] for every mode n (FP64 on the host)."""
+    return [np.einsum("ir,ir->r", ctx.mttkrp(n).astype(np.float64), Y[n].astype(np.float64))
+            for n in range(N)]
+
+
+def test_c3_full_mixed_equal_fit():
+    say("c3 full: generate")
+    T, _, _ = host.generate((1_000_000, 200_000, 20_000, 500), 150_000_000, seed=3, coords="zipf",
+                            alpha=1.2, values="lognormal", threads=16)
+    R, iters = 64, 3
+    Y0 = init_factors(T.dims, R, 42, "normal")
+    with GpuContext(0) as ctx:
+        say("c3 full: upload + remap")
+        ctx.upload(T)
+        ctx.remap_all()
+        # positive factors: the column sums have no cancellation
+        rng = np.random.default_rng(0)
+        U = [rng.random((d, R), dtype=np.float32) for d in T.dims]
+        ctx.set_factors(U, R)
+        cols = column_products(ctx, U, 4)
+        for n in range(1, 4):
+            np.testing.assert_allclose(cols[n], cols[0], rtol=5e-5)
+        ctx.set_factors(Y0, R)
+        say("c3 full: cpals plain / bf16")
+        plain = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_FP32)
+        ctx.set_factors(Y0, R)
+        mixed = ctx.run_cpals(iters, 0.0, precision=L.DYN_PREC_BF16)
+    say("fits", list(plain.fit_history), list(mixed.fit_history))
+    assert np.all(np.isfinite(plain.fit_history)) and plain.fit_history[-1] > plain.fit_history[0]
+    np.testing.assert_allclose(mixed.fit_history, plain.fit_history, rtol=0, atol=1e-4)
+
+
+def test_c4_full_remap_invariant_and_jacobi():
+    say("c4 full: generate")
+    T, _, _ = host.generate((500_000, 100_000, 50_000, 5_000, 100), 400_000_000, seed=4,
+                            coords="zipf", alpha=1.0, values="planted", planted_rank=20,
+                            noise_frac=0.05, threads=16)
+    R = 32
+    Y0 = init_factors(T.dims, R, 42, "normal")
+    with GpuContext(0) as ctx:
+        say("c4 full: upload + remap")
+        ctx.upload(T)
+        for mode in range(5):
+            rowptr, _ = ctx.remap(mode, 1 if mode == 0 else 0)
+            counts = np.bincount(T.idx[mode], minlength=T.dims[mode])
+            assert rowptr[0] == 0 and rowptr[-1] == T.nnz
+            np.testing.assert_array_equal(np.diff(rowptr.astype(np.int64)), counts)
+        rng = np.random.default_rng(0)
+        U = [rng.random((d, R), dtype=np.float32) for d in T.dims]
+        ctx.set_factors(U, R)
+        say("c4 full: column invariant")
+        cols = column_products(ctx, U, 5)
+        for n in range(1, 5):
+            np.testing.assert_allclose(cols[n], cols[0], rtol=5e-5)
+        say("c4 full: one Jacobi and one Gauss-Seidel sweep")
+        ctx.set_factors(Y0, R)
+        jac = ctx.run_cpals(1, 0.0, kernel=L.DYN_CP_JACOBI)
+        lam_j = ctx.get_lambda()
+        ctx.set_factors(Y0, R)
+        gs = ctx.run_cpals(1, 0.0, kernel=L.DYN_CP_GAUSS_SEIDEL)
+    assert np.isfinite(jac.final_fit) and np.isfinite(gs.final_fit)
+    assert np.all(np.isfinite(lam_j)) and np.all(lam_j >= 0)
+
+
+def test_c5_full_completion_adam_and_gradients():
+    torch = pytest.importorskip("torch")
+    from dynasor_plus_amd.autograd import completion_loss_and_grad, fit_completion
+    say("c5 full: generate")
+    T, _, _ = host.generate((1_000_000, 100_000, 2_000), 200_000_000, seed=5, coords="uniform",
+                            values="bernoulli", planted_rank=16, planted_factors="normal",
+                            planted_var=0.5, threads=16)
+    orc = load_oracle()
+    steps, R = 100, 32
+    with GpuContext(0) as eng:
+        say("c5 full: upload")
+        eng.upload(T)
+        say("c5 full: adam")
+        A, losses = fit_completion(eng, T.dims, rank=R, steps=steps, lr=1e-2, seed=0)
+        final, grads = completion_loss_and_grad(eng, A, "bernoulli")
+        A64 = [a.cpu().numpy().astype(np.float64) for a in A]
+        G = [g.cpu().numpy() for g in grads]
+    assert len(losses) == steps and np.all(np.isfinite(losses))
+    assert losses[-1] < 0.95 * losses[0]
+    say("c5 full: oracle loss")
+    L64 = orc.completion_loss(T, A64, 1)
+    assert abs(final - L64) / L64 < 1e-5, (final, L64)
+    say("c5 full: restricted central differences")
+    rng = np.random.default_rng(7)
+    h = 1e-4
+    fd, ad = [], []
+    y = T.vals.astype(np.float64)
+    for _ in range(12):
+        n = int(rng.integers(0, 3))
+        i = int(rng.integers(0, T.dims[n]))
+        r = int(rng.integers(0, R))
+        k = np.nonzero(T.idx[n] == i)[0]
+        if k.size == 0:
+            continue
+        rows = [A64[w][T.idx[w][k]] for w in range(3)]
+
+        def loss_at(delta):
+            rr = [x.copy() for x in rows]
+            rr[n][:, r] += delta
+            m = rr[0] * rr[1] * rr[2]
+            m = m.sum(axis=1)
+            return float(np.sum(np.logaddexp(0.0, m) - y[k] * m))
+
+        fd.append((loss_at(h) - loss_at(-h)) / (2 * h))
+        ad.append(float(G[n][i, r]))
+    fd, ad = np.array(fd), np.array(ad)
+    assert fd.size >= 8
+    assert np.linalg.norm(ad - fd) / np.linalg.norm(fd) < 1e-4, (ad, fd)
