@@ -30,10 +30,12 @@
 #include "dyn_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace dyn {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kBufDword3 = 0x00020000;   // raw buffer resource word 3 on gfx9-family parts
 constexpr int kQuads = kBlock / 4;            // quads per 256-thread block
 
 struct SegArgs {
@@ -55,6 +57,7 @@ struct SegArgs {
     float*          part;         // [npieces][RP]
     double*         loss_dev;     // completion: summed loss (device scalar)
     uint32_t        nwaves;       // waves of the launch (set by the launcher)
+    uint32_t        stream_bytes; // bytes of one value / index column (buffer range)
     // completion kernel only
     const float*    An;           // output-mode factor rows (FP32, input order)
     double*         loss_part;    // [npieces + nseg] or null
@@ -220,6 +223,18 @@ __device__ __forceinline__ void bcast_batch(float v_l, const uint32_t* i_l, floa
     v = qbf<J>(v_l);
 #pragma unroll
     for (int w = 0; w < NO; ++w) iw[w] = qb<J>(i_l[w]);
+}
+
+// Round T of a heavy piece: nonzero j of the round is component j of lane T.
+template <int T, int NO>
+__device__ __forceinline__ void bcast_round(const float* sv4, const uint32_t (*si4)[4], float* vb,
+                                            uint32_t (*ib)[NO]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        vb[j] = qbf<T>(sv4[j]);
+#pragma unroll
+        for (int w = 0; w < NO; ++w) ib[j][w] = qb<T>(si4[w][j]);
+    }
 }
 
 // A factor row by frequency rank (raw words of this lane): from the LDS copy
@@ -388,6 +403,14 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     // sum is flushed into the slot of the last piece of each row it touches,
     // which is where k_heavy looks for it.
     {
+        // buffer resources over the copy's value / index columns (dword ranges)
+        __amdgpu_buffer_rsrc_t rs_idx[NO];
+#pragma unroll
+        for (int w = 0; w < NO; ++w)
+            rs_idx[w] = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(a.oidx[w]), (short)0,
+                                                          (int)a.stream_bytes, kBufDword3);
+        const __amdgpu_buffer_rsrc_t rs_val = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(a.vals), (short)0, (int)a.stream_bytes, kBufDword3);
         const uint32_t pq = a.npieces / nwaves, prem = a.npieces % nwaves;
         const uint32_t p0 = range_begin(gw, pq, prem);
         const uint32_t p1 = range_begin(gw + 1, pq, prem);
@@ -410,35 +433,48 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 if constexpr (KIND == 1) load_row<RP, float>(a.An + (size_t)row * RP, q, arow);
             }
             // the piece's stream: quad g owns nonzeros [16 g, 16 g + 16), lane q
-            // of it holds 16 g + 4 t + q for t < kPieceLane
-            float sv[PL];
-            uint32_t si[PL][NO];
+            // of it holds the 4 consecutive ones from 16 g + 4 q (one 16-byte
+            // buffer load per column; dword-aligned offsets)
+            static_assert(PL == 4, "a lane holds 4 consecutive nonzeros of its quad's 16");
+            const uint32_t e0 = (uint32_t)(16 * qw + 4 * q);
+            float sv4[4];
+            uint32_t si4[NO][4];
+            {
+                const uint32_t boff = (kb + e0) * 4u;
 #pragma unroll
-            for (int t = 0; t < PL; ++t) {
-                const uint32_t k = (uint32_t)(16 * qw + 4 * t + q);
-                const bool in = k < n;
-                sv[t] = in ? (a.vperm ? a.vals[a.vperm[kb + k]] : a.vals[kb + k]) : 0.f;
+                for (int w = 0; w < NO; ++w) {
+                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_idx[w], boff, 0, 0);
 #pragma unroll
-                for (int w = 0; w < NO; ++w) si[t][w] = in ? a.oidx[w][kb + k] : 0u;
+                    for (int c = 0; c < 4; ++c) si4[w][c] = (e0 + c < n) ? x[c] : 0u;
+                }
+                if (a.vperm) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        sv4[c] = (e0 + c < n) ? a.vals[a.vperm[kb + e0 + c]] : 0.f;
+                } else {
+                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_val, boff, 0, 0);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) sv4[c] = (e0 + c < n) ? __uint_as_float(x[c]) : 0.f;
+                }
             }
-#pragma unroll
-            for (int t = 0; t < PL; t += NR) {
-                const uint32_t k0 = (uint32_t)(16 * qw + 4 * t);
+            // round t: nonzeros 16 g + 4 t + j, held by lane t of the quad
+            auto do_rounds = [&](auto t0c) {
+                constexpr int T0 = decltype(t0c)::value;
+                const uint32_t k0 = (uint32_t)(16 * qw + 4 * T0);
                 if (k0 < n) {
                     float vb[4 * NR];
                     uint32_t ib[4 * NR][NO];
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) {
-                        bcast_batch<0, NO>(sv[t + r], si[t + r], vb[4 * r + 0], ib[4 * r + 0]);
-                        bcast_batch<1, NO>(sv[t + r], si[t + r], vb[4 * r + 1], ib[4 * r + 1]);
-                        bcast_batch<2, NO>(sv[t + r], si[t + r], vb[4 * r + 2], ib[4 * r + 2]);
-                        bcast_batch<3, NO>(sv[t + r], si[t + r], vb[4 * r + 3], ib[4 * r + 3]);
-                    }
+                    bcast_round<T0, NO>(sv4, si4, vb, ib);
+                    if constexpr (NR > 1) bcast_round<T0 + 1, NO>(sv4, si4, vb + 4, ib + 4);
                     batch4<RP, NO, FT, KIND, 4 * NR, (NR > 1 ? 4 * NR : HB), CPL>(
                         a, gb, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
                         [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
                 }
-            }
+            };
+            do_rounds(std::integral_constant<int, 0>{});
+            if constexpr (NR == 1) do_rounds(std::integral_constant<int, 1>{});
+            do_rounds(std::integral_constant<int, 2>{});
+            if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{});
             const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
             const bool flush = row_ends || (p + 1 == p1);
             if (flush) {
@@ -671,6 +707,7 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     a->pc_e = cp.pc_e;
     a->pc_row = cp.pc_row;
     a->npieces = cp.npieces;
+    a->stream_bytes = (uint32_t)std::min<uint64_t>(c->nnz * 4ull, 0xFFFFFFF0ull);
     void* sp = nullptr;
     const size_t pieces = std::max<uint32_t>(cp.npieces, 1);
     const size_t pbytes = ((pieces * Rp * sizeof(float)) + 255) & ~size_t(255);
