@@ -667,7 +667,7 @@ static int solve(dyn_gpu_ctx* c, int n, SolveParts* out) {
 }
 
 static int fit_eval(dyn_gpu_ctx* c, const double* ipart, uint32_t nwg, const double* inner_dev,
-                    double tnorm2) {
+                    double tnorm2, double* out) {
     FitArgs a;
     for (int m = 0; m < c->N; ++m) a.G[m] = c->G[m];
     a.N = c->N;
@@ -678,7 +678,7 @@ static int fit_eval(dyn_gpu_ctx* c, const double* ipart, uint32_t nwg, const dou
     a.nwg = nwg;
     a.inner_dev = inner_dev;
     a.tnorm2 = tnorm2;
-    a.out = c->fitbuf;
+    a.out = out;
     hipLaunchKernelGGL(k_fit, dim3(1), dim3(kBlock), 0, c->stream, a);
     return check_launch(c, "fit");
 }
@@ -742,33 +742,62 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                     o->tol, jacobi ? "jacobi" : "gauss-seidel", bf ? "bf16-gather" : "fp32");
     }
 
-    hipEvent_t ev[2 * kMaxModes];
-    for (int i = 0; i < 2 * kMaxModes; ++i) DYN_HIP(c, hipEventCreateWithFlags(&ev[i], 0));
+    // With tol <= 0 the run never stops early, so the fits stay on the device
+    // and are read once at the end (no host round trip per iteration); the
+    // per-iteration times then come from events.
+    const int iters = std::max(o->max_iters, 0);
+    const bool deferred = !(o->tol > 0.0) && iters > 0;
+    const int per = 2 * N + 1;   // iteration start, then a begin/end pair per spMTTKRP
+    std::vector<hipEvent_t> evs((size_t)(deferred ? iters : 1) * per + 1, nullptr);
+    double* hist_dev = nullptr;
     // the Hadamard / Cholesky / inverse of mode n depends only on the Grams,
     // so it runs on the auxiliary stream while mode n's spMTTKRP runs
-    hipEvent_t ev_g, ev_l;
-    DYN_HIP(c, hipEventCreateWithFlags(&ev_g, hipEventDisableTiming));
-    DYN_HIP(c, hipEventCreateWithFlags(&ev_l, hipEventDisableTiming));
+    hipEvent_t ev_g = nullptr, ev_l = nullptr;
     auto destroy_events = [&]() {
-        for (int i = 0; i < 2 * kMaxModes; ++i) (void)hipEventDestroy(ev[i]);
-        (void)hipEventDestroy(ev_g);
-        (void)hipEventDestroy(ev_l);
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_g) (void)hipEventDestroy(ev_g);
+        if (ev_l) (void)hipEventDestroy(ev_l);
+        if (hist_dev) dfree(c, hist_dev);
     };
+    {
+        bool ok = hipEventCreateWithFlags(&ev_g, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&ev_l, hipEventDisableTiming) == hipSuccess;
+        for (size_t i = 0; ok && i < evs.size(); ++i)
+            ok = hipEventCreateWithFlags(&evs[i], 0) == hipSuccess;
+        int arc = ok ? DYN_OK : set_err(c, DYN_ERR_HIP, "event creation failed");
+        if (arc == DYN_OK && deferred) arc = dalloc_t(c, &hist_dev, (size_t)iters * 4);
+        if (arc != DYN_OK) {
+            destroy_events();
+            return arc;
+        }
+    }
 
     int rc = DYN_OK;
     double fit_prev = 0.0;
     auto t_start = clk::now();
     double* inner_dev = c->fitbuf + 8;
-    for (int iter = 0; iter < o->max_iters && rc == DYN_OK; ++iter) {
+    auto mttkrp_secs = [&](const hipEvent_t* E, int nev) {
+        double mt = 0.0;
+        for (int i = 0; i < nev; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, E[1 + 2 * i], E[2 + 2 * i]) == hipSuccess) mt += ms * 1e-3;
+        }
+        return mt;
+    };
+    int nev = 0;
+    for (int iter = 0; iter < iters && rc == DYN_OK; ++iter) {
         auto t0 = clk::now();
+        hipEvent_t* E = &evs[(size_t)(deferred ? iter : 0) * per];
+        (void)hipEventRecord(E[0], c->stream);
         const void* Yin[kMaxModes];
         for (int m = 0; m < N; ++m) Yin[m] = bf ? (const void*)c->Ygb[m] : (const void*)c->Yg[m];
-        int nev = 0;
+        nev = 0;
         if (jacobi) {
-            (void)hipEventRecord(ev[0], c->stream);
+            (void)hipEventRecord(E[1], c->stream);
             for (int m = 0; m < N && rc == DYN_OK; ++m)
                 rc = mttkrp_mode(c, m, o->precision, Yin, nullptr, c->Yhat[m], c->Rp);
-            (void)hipEventRecord(ev[1], c->stream);
+            (void)hipEventRecord(E[2], c->stream);
             nev = 1;
         }
         double* ipart = nullptr;
@@ -786,9 +815,9 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                 break;
             }
             if (!jacobi) {
-                (void)hipEventRecord(ev[2 * n], c->stream);
+                (void)hipEventRecord(E[1 + 2 * n], c->stream);
                 rc = mttkrp_mode(c, n, o->precision, Yin, nullptr, c->Yhat[n], c->Rp);
-                (void)hipEventRecord(ev[2 * n + 1], c->stream);
+                (void)hipEventRecord(E[2 + 2 * n], c->stream);
                 nev = n + 1;
                 if (rc != DYN_OK) break;
             }
@@ -809,7 +838,9 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
             rc = forward_eval(c, Ys, c->Rp, c->lambda, nullptr, c->vals, inner_dev);
             if (rc != DYN_OK) break;
         }
-        if ((rc = fit_eval(c, ipart, inwg, inner_dev, tn2)) != DYN_OK) break;
+        double* fit_out = deferred ? hist_dev + (size_t)iter * 4 : c->fitbuf;
+        if ((rc = fit_eval(c, ipart, inwg, inner_dev, tn2, fit_out)) != DYN_OK) break;
+        if (deferred) continue;
         hipError_t e = hipMemcpyAsync(c->fitbuf_host, c->fitbuf, 4 * sizeof(double),
                                       hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -817,11 +848,6 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
             rc = set_err(c, DYN_ERR_HIP, "cpals iteration %d failed: %s", iter,
                          hipGetErrorString(e));
             break;
-        }
-        double mt = 0.0;
-        for (int i = 0; i < nev; ++i) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) mt += ms * 1e-3;
         }
         const double fit = c->fitbuf_host[0];
         const double resid = c->fitbuf_host[1];
@@ -831,7 +857,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         res->final_fit = fit;
         res->final_residual = resid;
         res->iters_done = iter + 1;
-        res->mttkrp_time_s += mt;
+        res->mttkrp_time_s += mttkrp_secs(E, nev);
         if (o->verbose) {
             std::printf("[cpals] iter %3d  fit = %.6f  dfit = %.2e  resid = %.3e  iter = %.3f s\n",
                         iter, fit, iter == 0 ? 0.0 : std::fabs(fit - fit_prev), resid, it);
@@ -843,6 +869,38 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
             break;
         }
         fit_prev = fit;
+    }
+    if (deferred && rc == DYN_OK) {
+        hipEvent_t E_end = evs.back();
+        std::vector<double> h((size_t)iters * 4);
+        hipError_t e = hipEventRecord(E_end, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h.data(), hist_dev, h.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = set_err(c, DYN_ERR_HIP, "cpals failed: %s", hipGetErrorString(e));
+        } else {
+            for (int iter = 0; iter < iters; ++iter) {
+                const hipEvent_t* E = &evs[(size_t)iter * per];
+                const hipEvent_t next = (iter + 1 < iters) ? evs[(size_t)(iter + 1) * per] : E_end;
+                float ms = 0.f;
+                const double it =
+                    hipEventElapsedTime(&ms, E[0], next) == hipSuccess ? ms * 1e-3 : 0.0;
+                const double fit = h[(size_t)iter * 4], resid = h[(size_t)iter * 4 + 1];
+                if (res->fit_history) res->fit_history[iter] = fit;
+                if (res->iter_time_s) res->iter_time_s[iter] = it;
+                res->mttkrp_time_s += mttkrp_secs(E, nev);
+                res->final_fit = fit;
+                res->final_residual = resid;
+                res->iters_done = iter + 1;
+                if (o->verbose)
+                    std::printf("[cpals] iter %3d  fit = %.6f  dfit = %.2e  resid = %.3e  "
+                                "iter = %.3f s\n",
+                                iter, fit, iter == 0 ? 0.0 : std::fabs(fit - fit_prev), resid, it);
+                fit_prev = fit;
+            }
+        }
     }
     destroy_events();
     res->total_time_s = secs(t_start, clk::now());
