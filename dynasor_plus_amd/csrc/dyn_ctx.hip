@@ -160,6 +160,7 @@ void free_copy(dyn_gpu_ctx* c, int m) {
     dfree(c, cp.pc_k);
     dfree(c, cp.pc_e);
     dfree(c, cp.pc_row);
+    dfree(c, cp.pc_hot);
     dfree(c, cp.heavy_row);
     dfree(c, cp.heavy_first);
     dfree(c, cp.heavy_count);

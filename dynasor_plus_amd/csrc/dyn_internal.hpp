@@ -57,6 +57,12 @@ struct ModeCopy {
     uint32_t* pc_k = nullptr;
     uint32_t* pc_e = nullptr;
     uint32_t* pc_row = nullptr;
+    uint32_t* pc_hot = nullptr;                 // null, or 1 when every nonzero of the
+                                                // piece has hot_w rank < hot_h, else 0
+    // hot split: inside every row, the nonzeros whose mode-hot_w rank is below
+    // hot_h come first (stable), and heavy pieces never straddle the split
+    int       hot_w = -1;
+    uint32_t  hot_h = 0;
     uint32_t  nheavy = 0;
     uint32_t* heavy_row = nullptr;              // nheavy
     uint32_t* heavy_first = nullptr;            // nheavy : first piece
@@ -168,6 +174,11 @@ int check_launch(dyn_gpu_ctx* c, const char* what);
 // building blocks implemented in the .hip units -----------------------------
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary);
 void free_copy(dyn_gpu_ctx* c, int mode);
+// Reorders copy `mode` so that in every row the nonzeros with relabelled
+// mode-w index < H precede the others (stable within both classes) and
+// rebuilds its work tables with hot/cold pieces.  No-op when already split
+// for (w, H).
+int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H);
 
 // Segmented spMTTKRP of mode `mode` from the per-mode copy into `out`
 // (I_mode x Rp floats, device).  `Yin[w]` are the gathered factors (FP32 or

@@ -58,6 +58,7 @@ struct SegArgs {
     double*         loss_dev;     // completion: summed loss (device scalar)
     uint32_t        nwaves;       // waves of the launch (set by the launcher)
     uint32_t        stream_bytes; // bytes of one value / index column (buffer range)
+    const uint32_t* pc_hot;       // hot split of operand 1 (null: none)
     // completion kernel only
     const float*    An;           // output-mode factor rows (FP32, input order)
     double*         loss_part;    // [npieces + nseg] or null
@@ -246,12 +247,57 @@ __device__ __forceinline__ void gather_raw(const FT* gq, const FT* lq, uint32_t 
     load_raw<RP, FT>(((i < hot) ? lq : gq) + (size_t)i * RP, r);
 }
 
+// The same load from a known address space: LDS (ds_read) or global memory.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+template <int RP, typename FT, int AS>
+__device__ __forceinline__ void load_raw_as(const FT* lane_row, uint32_t* r) {
+    using C = Cols<RP, FT>;
+    typedef __attribute__((address_space(AS))) const u32x4_t v4p;
+    typedef __attribute__((address_space(AS))) const u32x2_t v2p;
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b) {
+        if constexpr (C::LW * sizeof(FT) == 16) {
+            const u32x4_t x = *(v4p*)(lane_row + b * 4 * C::LW);
+            r[b * 4 + 0] = x.x;
+            r[b * 4 + 1] = x.y;
+            r[b * 4 + 2] = x.z;
+            r[b * 4 + 3] = x.w;
+        } else {
+            const u32x2_t x = *(v2p*)(lane_row + b * 4 * C::LW);
+            r[b * 2 + 0] = x.x;
+            r[b * 2 + 1] = x.y;
+        }
+    }
+}
+
 // per-operand gather bases of one lane
 template <int NO, typename FT>
 struct GatherBase {
     const FT* g[NO];
     const FT* l[NO];
 };
+
+// Whether heavy pieces take the hot-split path (FP32 operands, N >= 3).
+template <int NO, typename FT>
+constexpr bool kSplitPath() { return NO >= 2 && sizeof(FT) == 4; }
+
+// Gathers of operand w.  SPLIT 0: no hot split (LDS or memory by rank).
+// SPLIT 1: a hot-split piece -- operand 1 comes from LDS when the piece is
+// hot (`hot1`, uniform over the wave) and from memory otherwise; operand 0
+// (reused across a fiber) from LDS or memory by rank; the others from memory.
+template <int RP, typename FT, int SPLIT>
+__device__ __forceinline__ void gather_op(const SegArgs& a, bool hot1, int w, const FT* gq,
+                                          const FT* lq, uint32_t i, uint32_t* r) {
+    if constexpr (SPLIT == 0) {
+        gather_raw<RP, FT>(gq, lq, a.hot[w], i, r);
+    } else {
+        if (w == 0) gather_raw<RP, FT>(gq, lq, a.hot[0], i, r);
+        else if (w == 1 && hot1) load_raw_as<RP, FT, 3>(lq + (size_t)i * RP, r);
+        else load_raw_as<RP, FT, 1>(gq + (size_t)i * RP, r);
+    }
+}
 
 // One nonzero's contribution to the quad's accumulator (y[w] = gathered rows).
 // KIND 0 forms v*y0, times the middle operands, then a fused multiply-add
@@ -299,13 +345,13 @@ __device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)
 // index repeats), then their accumulation in order.  `live(j)` says whether
 // nonzero j of the batch exists; `before_step(j)` runs ahead of its
 // accumulation (row changes of light segments).
-template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, typename Live,
-          typename Before>
+template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, int SPLIT = 0,
+          typename Live, typename Before>
 __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT>& gb,
                                        const float* vb, const uint32_t (*ib)[NO], int q,
                                        float* acc, const float* arow, double* lsum,
                                        uint32_t& prev0, uint32_t* raw0prev, Live live,
-                                       Before before_step) {
+                                       Before before_step, bool hot1 = false) {
     constexpr int RD = Cols<RP, FT>::RD;
 #pragma unroll
     for (int j0 = 0; j0 < J; j0 += HB) {
@@ -316,10 +362,11 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT
             const int j = j0 + h;
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
             reuse[h] = (ib[j][0] == before);
-            if (!reuse[h]) gather_raw<RP, FT>(gb.g[0], gb.l[0], a.hot[0], ib[j][0], raw[h][0]);
+            if (!reuse[h])
+                gather_op<RP, FT, SPLIT>(a, hot1, 0, gb.g[0], gb.l[0], ib[j][0], raw[h][0]);
 #pragma unroll
             for (int w = 1; w < NO; ++w)
-                gather_raw<RP, FT>(gb.g[w], gb.l[w], a.hot[w], ib[j][w], raw[h][w]);
+                gather_op<RP, FT, SPLIT>(a, hot1, w, gb.g[w], gb.l[w], ib[j][w], raw[h][w]);
         }
 #pragma unroll
         for (int h = 0; h < HB; ++h) {
@@ -458,23 +505,30 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 }
             }
             // round t: nonzeros 16 g + 4 t + j, held by lane t of the quad
-            auto do_rounds = [&](auto t0c) {
+            // a hot-split piece reads operand 1 from LDS (hot) or memory (cold)
+            const bool hot1 = kSplitPath<NO, FT>() && a.pc_hot[p] != 0u;
+            auto do_rounds = [&](auto t0c, auto splitc) {
                 constexpr int T0 = decltype(t0c)::value;
+                constexpr int SP = decltype(splitc)::value;
                 const uint32_t k0 = (uint32_t)(16 * qw + 4 * T0);
                 if (k0 < n) {
                     float vb[4 * NR];
                     uint32_t ib[4 * NR][NO];
                     bcast_round<T0, NO>(sv4, si4, vb, ib);
                     if constexpr (NR > 1) bcast_round<T0 + 1, NO>(sv4, si4, vb + 4, ib + 4);
-                    batch4<RP, NO, FT, KIND, 4 * NR, (NR > 1 ? 4 * NR : HB), CPL>(
+                    batch4<RP, NO, FT, KIND, 4 * NR, (NR > 1 ? 4 * NR : HB), CPL, SP>(
                         a, gb, vb, ib, q, acc, arow, &lsum, prev0, raw0prev,
-                        [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {});
+                        [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {}, hot1);
                 }
             };
-            do_rounds(std::integral_constant<int, 0>{});
-            if constexpr (NR == 1) do_rounds(std::integral_constant<int, 1>{});
-            do_rounds(std::integral_constant<int, 2>{});
-            if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{});
+            auto do_piece = [&](auto splitc) {
+                do_rounds(std::integral_constant<int, 0>{}, splitc);
+                if constexpr (NR == 1) do_rounds(std::integral_constant<int, 1>{}, splitc);
+                do_rounds(std::integral_constant<int, 2>{}, splitc);
+                if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{}, splitc);
+            };
+            // FP32 launches on three or more modes run on a hot-split copy
+            do_piece(std::integral_constant<int, kSplitPath<NO, FT>() ? 1 : 0>{});
             const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
             const bool flush = row_ends || (p + 1 == p1);
             if (flush) {
@@ -617,16 +671,51 @@ __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part
 // ---------------------------------------------------------------------------
 //  host-side dispatch
 // ---------------------------------------------------------------------------
+// Rows of operand 1 held in LDS under a hot split (the whole budget).
+static uint32_t hot_split_rows(dyn_gpu_ctx* c, int Rp, int elem_bytes, uint32_t rows) {
+    const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
+    return (uint32_t)std::min<size_t>(rows, budget / ((size_t)Rp * elem_bytes));
+}
+
+// The hot split used by FP32-operand spMTTKRP launches on tensors with three
+// or more modes: operand 1 (the first mode after the secondary sort mode) is
+// split at the rows that fit in LDS for this rank padding.  (BF16 rows are
+// half as long: there the rank-ordered LDS rows of both operands, read
+// through generic loads, measured faster than the split.)
+static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int elem_bytes) {
+    if (c->N < 3) return DYN_OK;
+    if (elem_bytes != 4) {
+        // BF16 operands run on the plain (row, secondary) order: undo a split
+        const ModeCopy& cp = c->copy[mode];
+        return cp.hot_w >= 0 ? remap_mode(c, mode, cp.secondary) : DYN_OK;
+    }
+    int others[kMaxModes], j = 0;
+    for (int w = 0; w < c->N; ++w)
+        if (w != mode) others[j++] = w;
+    return ensure_hot_split(c, mode, others[1],
+                            hot_split_rows(c, Rp, elem_bytes, c->dims[others[1]]));
+}
+
 template <int RP, int NO, typename FT, int KIND>
 static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
     // hot rows: equal share of the LDS budget per gathered operand
     const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
     const size_t rowb = (size_t)RP * sizeof(FT);
-    const uint32_t per = (uint32_t)(budget / ((size_t)NO * rowb));
     size_t bytes = 0;
-    for (int w = 0; w < NO; ++w) {
-        a.hot[w] = std::min<uint32_t>(per, dims_o[w]);
-        bytes += (size_t)a.hot[w] * rowb;
+    if (kSplitPath<NO, FT>() && a.pc_hot) {
+        // hot split: operand 1's leading rows, exactly the split, then as many
+        // of operand 0's as the rest of the budget holds
+        for (int w = 0; w < NO; ++w) a.hot[w] = 0;
+        a.hot[1] = hot_split_rows(c, RP, (int)sizeof(FT), dims_o[1]);
+        a.hot[0] = std::min<uint32_t>(
+            dims_o[0], (uint32_t)((budget - std::min(budget, (size_t)a.hot[1] * rowb)) / rowb));
+        bytes = ((size_t)a.hot[0] + a.hot[1]) * rowb;
+    } else {
+        const uint32_t per = (uint32_t)(budget / ((size_t)NO * rowb));
+        for (int w = 0; w < NO; ++w) {
+            a.hot[w] = std::min<uint32_t>(per, dims_o[w]);
+            bytes += (size_t)a.hot[w] * rowb;
+        }
     }
     auto fn = k_seg<RP, NO, FT, KIND>;
     static bool attr_set = false;
@@ -706,6 +795,7 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     a->pc_k = cp.pc_k;
     a->pc_e = cp.pc_e;
     a->pc_row = cp.pc_row;
+    a->pc_hot = cp.pc_hot;
     a->npieces = cp.npieces;
     a->stream_bytes = (uint32_t)std::min<uint64_t>(c->nnz * 4ull, 0xFFFFFFF0ull);
     void* sp = nullptr;
@@ -725,6 +815,8 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
     SegArgs a;
     int NO = 0;
     uint32_t dims_o[kMaxModes];
+    if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
+    DYN_CHECK(prepare_split(c, mode, Rp, prec == DYN_PREC_BF16 ? 2 : 4));
     DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false));
     if (vals_override) {
         a.vals = vals_override;
@@ -936,6 +1028,8 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         int NO = 0;
         uint32_t dims_o[kMaxModes];
         const bool want_loss = (mode == 0 && loss_out);
+        if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
+        DYN_CHECK(prepare_split(c, mode, Rp, 4));
         DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, want_loss));
         a.out = grad[mode];
         a.An = A[mode];

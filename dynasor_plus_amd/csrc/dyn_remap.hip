@@ -367,12 +367,23 @@ int ensure_gather(dyn_gpu_ctx* c) {
 
 // Work tables from the row pointer (host): heavy rows cut into kPiece
 // pieces, light rows grouped by the kLightWindow window of their first
-// nonzero.
-static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32_t>& rp) {
+// nonzero.  With per-row hot counts `nh` (a hot split), a heavy row's pieces
+// are cut separately in its hot prefix and its cold rest, and each piece
+// records which class it holds.
+static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32_t>& rp,
+                          const std::vector<uint32_t>* nh = nullptr) {
     const uint32_t dim = (uint32_t)rp.size() - 1;
-    std::vector<uint32_t> sk, se, sr, pk, pe, pr, hr, hf, hc;
+    std::vector<uint32_t> sk, se, sr, pk, pe, pr, ph, hr, hf, hc;
     bool open = false;
     uint32_t window = 0;
+    auto cut = [&](uint32_t r, uint32_t b, uint32_t e, uint32_t hot) {
+        for (uint32_t k = b; k < e; k += kPiece) {
+            pk.push_back(k);
+            pe.push_back(std::min(k + kPiece, e));
+            pr.push_back(r);
+            ph.push_back(hot);
+        }
+    };
     for (uint32_t r = 0; r < dim; ++r) {
         const uint32_t L = rp[r + 1] - rp[r];
         if (L == 0) continue;
@@ -381,15 +392,16 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
                 se.push_back(rp[r]);
                 open = false;
             }
-            const uint32_t np = (L + kPiece - 1) / kPiece;
             hr.push_back(r);
             hf.push_back((uint32_t)pk.size());
-            hc.push_back(np);
-            for (uint32_t j = 0; j < np; ++j) {
-                pk.push_back(rp[r] + j * kPiece);
-                pe.push_back(std::min(rp[r] + (j + 1) * kPiece, rp[r + 1]));
-                pr.push_back(r);
+            if (nh) {
+                const uint32_t split = rp[r] + (*nh)[r];
+                cut(r, rp[r], split, 1u);
+                cut(r, split, rp[r + 1], 0u);
+            } else {
+                cut(r, rp[r], rp[r + 1], 0u);
             }
+            hc.push_back((uint32_t)pk.size() - hf.back());
         } else {
             const uint32_t wdw = rp[r] / kLightWindow;
             if (!open || wdw != window) {
@@ -402,6 +414,11 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
         }
     }
     if (open) se.push_back(rp[dim]);
+    for (uint32_t** t : {&cp.seg_k, &cp.seg_e, &cp.seg_row, &cp.pc_k, &cp.pc_e, &cp.pc_row,
+                         &cp.pc_hot, &cp.heavy_row, &cp.heavy_first, &cp.heavy_count}) {
+        dfree(c, *t);
+        *t = nullptr;
+    }
     cp.nseg = (uint32_t)sk.size();
     cp.npieces = (uint32_t)pk.size();
     cp.nheavy = (uint32_t)hr.size();
@@ -417,10 +434,142 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
     DYN_CHECK(up(&cp.pc_k, pk));
     DYN_CHECK(up(&cp.pc_e, pe));
     DYN_CHECK(up(&cp.pc_row, pr));
+    if (nh) DYN_CHECK(up(&cp.pc_hot, ph));
     DYN_CHECK(up(&cp.heavy_row, hr));
     DYN_CHECK(up(&cp.heavy_first, hf));
     DYN_CHECK(up(&cp.heavy_count, hc));
     DYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
+// ---------------------------------------------------------------------------
+//  hot split of a sorted copy (see ensure_hot_split)
+// ---------------------------------------------------------------------------
+__global__ void k_hot_flag(const uint32_t* __restrict__ ix, uint32_t n, uint32_t H,
+                           uint32_t* __restrict__ f) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += stride)
+        f[k] = (k < n && ix[k] < H) ? 1u : 0u;
+}
+
+// per row: hot count = hs[rp[r+1]] - hs[rp[r]] (hs: exclusive scan of the flags)
+__global__ void k_row_hot(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ hs,
+                          uint32_t dim, uint32_t* __restrict__ nh) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+        nh[r] = hs[rp[r + 1]] - hs[rp[r]];
+}
+
+struct HotScatter {
+    const uint32_t* rp;
+    const uint32_t* hs;
+    uint32_t        dim;
+    uint32_t        n;
+    uint32_t        H;
+    const uint32_t* key;                    // the split mode's index column
+    const float*    vals_in;
+    float*          vals_out;
+    const uint32_t* perm_in;
+    uint32_t*       perm_out;
+    const uint32_t* idx_in[kMaxModes];
+    uint32_t*       idx_out[kMaxModes];
+    int             ncols;
+};
+
+__global__ void k_hot_scatter(HotScatter a) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+        // row of k: the last r with rp[r] <= k
+        uint32_t lo = 0, hi = a.dim;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.rp[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t b = a.rp[lo], e = a.rp[lo + 1];
+        const uint32_t hb = a.hs[b];
+        const uint32_t nh = a.hs[e] - hb;
+        const uint32_t hk = a.hs[k] - hb;
+        const uint32_t pos = (a.key[k] < a.H) ? b + hk : b + nh + (k - b - hk);
+        a.vals_out[pos] = a.vals_in[k];
+        a.perm_out[pos] = a.perm_in[k];
+        for (int j = 0; j < a.ncols; ++j) a.idx_out[j][pos] = a.idx_in[j][k];
+    }
+}
+
+int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
+    ModeCopy& cp = c->copy[mode];
+    if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
+    if (cp.hot_w == w && cp.hot_h == H) return DYN_OK;
+    const uint32_t n = (uint32_t)c->nnz;
+    const uint32_t dim = c->dims[mode];
+    // flags, exclusive scan over n + 1 entries (entry n = the hot total)
+    uint32_t* hs = nullptr;
+    uint32_t* nh_dev = nullptr;
+    DYN_CHECK(dalloc_t(c, &hs, (size_t)n + 1));
+    DYN_CHECK(dalloc_t(c, &nh_dev, std::max<uint32_t>(dim, 1)));
+    void* sp2 = nullptr;
+    const uint64_t nb = ((uint64_t)n + 1 + kScanTile - 1) / kScanTile;
+    DYN_CHECK(scratch2(c, (nb + 16) * sizeof(uint32_t), &sp2));
+    hipLaunchKernelGGL(k_hot_flag, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
+                       cp.idx[w], n, H, hs);
+    DYN_CHECK(check_launch(c, "hot_flag"));
+    DYN_CHECK(exclusive_scan(c, hs, (uint64_t)n + 1, (uint32_t*)sp2));
+    // scatter every column into fresh arrays
+    HotScatter a;
+    a.rp = cp.rowptr;
+    a.hs = hs;
+    a.dim = dim;
+    a.n = n;
+    a.H = H;
+    a.key = cp.idx[w];
+    a.vals_in = cp.vals;
+    a.perm_in = cp.perm;
+    float* nv = nullptr;
+    uint32_t* np = nullptr;
+    uint32_t* ni[kMaxModes] = {nullptr};
+    DYN_CHECK(dalloc_t(c, &nv, std::max<uint32_t>(n, 1)));
+    DYN_CHECK(dalloc_t(c, &np, std::max<uint32_t>(n, 1)));
+    a.vals_out = nv;
+    a.perm_out = np;
+    a.ncols = 0;
+    for (int v = 0; v < c->N; ++v) {
+        if (v == mode) continue;
+        DYN_CHECK(dalloc_t(c, &ni[v], std::max<uint32_t>(n, 1)));
+        a.idx_in[a.ncols] = cp.idx[v];
+        a.idx_out[a.ncols] = ni[v];
+        ++a.ncols;
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_hot_scatter, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
+        DYN_CHECK(check_launch(c, "hot_scatter"));
+    }
+    if (dim) {
+        hipLaunchKernelGGL(k_row_hot, dim3(grid_for(dim)), dim3(kBlock), 0, c->stream, cp.rowptr,
+                           hs, dim, nh_dev);
+        DYN_CHECK(check_launch(c, "row_hot"));
+    }
+    std::vector<uint32_t> rp((size_t)dim + 1), nh(dim);
+    DYN_HIP(c, hipMemcpyAsync(rp.data(), cp.rowptr, rp.size() * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    if (dim)
+        DYN_HIP(c, hipMemcpyAsync(nh.data(), nh_dev, (size_t)dim * 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, cp.vals);
+    dfree(c, cp.perm);
+    cp.vals = nv;
+    cp.perm = np;
+    for (int v = 0; v < c->N; ++v) {
+        if (v == mode) continue;
+        dfree(c, cp.idx[v]);
+        cp.idx[v] = ni[v];
+    }
+    dfree(c, hs);
+    dfree(c, nh_dev);
+    DYN_CHECK(build_segments(c, cp, rp, &nh));
+    cp.hot_w = w;
+    cp.hot_h = H;
     return DYN_OK;
 }
 
