@@ -110,16 +110,20 @@ int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
 /* Build the mode-`mode` copy: a stable device radix sort of the nonzeros by
  * idx[mode] (ties keep input order), the gathered value / other-index
  * columns, the row pointer (rowptr[r]..rowptr[r+1] = nonzeros of row r) and
- * the equal-nnz chunk -> first-row segment table.  Optional host outputs:
- * rowptr_out[dims[mode]+1], perm_out[nnz] (sorted position -> input
+ * the work tables (heavy-row pieces, light-row segments).  Optional host
+ * outputs: rowptr_out[dims[mode]+1], perm_out[nnz] (sorted position -> input
  * position).  `secondary` >= 0 additionally orders each row by
- * idx[secondary] (composite key); -1 keeps input order within rows. */
+ * idx[secondary] (composite key); -1 keeps input order within rows.
+ * FP32-operand spMTTKRPs on three or more modes later reorder the nonzeros
+ * inside each row (hot split: those whose next-mode factor row is held in
+ * LDS first, both classes stable); the row pointer is unchanged. */
 int dyn_gpu_remap(dyn_gpu_ctx* ctx, int mode, int secondary,
                   uint64_t* rowptr_out, uint32_t* perm_out);
 /* Build every mode's copy (secondary key: the first other mode). */
 int dyn_gpu_remap_all(dyn_gpu_ctx* ctx);
-/* Copy the mode-`mode` sorted copy back: vals_out[nnz] and, for every mode w
- * != mode, idx_out[w][nnz] (idx_out[mode] is ignored and may be NULL). */
+/* Copy the mode-`mode` sorted copy back, in its current order (see the hot
+ * split above): vals_out[nnz] and, for every mode w != mode, idx_out[w][nnz]
+ * (idx_out[mode] is ignored and may be NULL). */
 int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
                      uint32_t* const* idx_out);
 
