@@ -279,9 +279,9 @@ struct GatherBase {
     const FT* l[NO];
 };
 
-// Whether heavy pieces take the hot-split path (FP32 operands, N >= 3).
+// Whether heavy pieces take the hot-split path (FP32 operands, N = 3).
 template <int NO, typename FT>
-constexpr bool kSplitPath() { return NO >= 2 && sizeof(FT) == 4; }
+constexpr bool kSplitPath() { return NO == 2 && sizeof(FT) == 4; }
 
 // Gathers of operand w.  SPLIT 0: no hot split (LDS or memory by rank).
 // SPLIT 1: a hot-split piece -- operand 1 comes from LDS when the piece is
@@ -527,7 +527,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 do_rounds(std::integral_constant<int, 2>{}, splitc);
                 if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{}, splitc);
             };
-            // FP32 launches on three or more modes run on a hot-split copy
+            // FP32 launches on three modes run on a hot-split copy
             do_piece(std::integral_constant<int, kSplitPath<NO, FT>() ? 1 : 0>{});
             const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
             const bool flush = row_ends || (p + 1 == p1);
@@ -677,15 +677,16 @@ static uint32_t hot_split_rows(dyn_gpu_ctx* c, int Rp, int elem_bytes, uint32_t 
     return (uint32_t)std::min<size_t>(rows, budget / ((size_t)Rp * elem_bytes));
 }
 
-// The hot split used by FP32-operand spMTTKRP launches on tensors with three
-// or more modes: operand 1 (the first mode after the secondary sort mode) is
-// split at the rows that fit in LDS for this rank padding.  (BF16 rows are
-// half as long: there the rank-ordered LDS rows of both operands, read
-// through generic loads, measured faster than the split.)
+// The hot split used by FP32-operand spMTTKRP launches on 3-mode tensors:
+// operand 1 (the mode after the secondary sort mode) is split at the rows
+// that fit in LDS for this rank padding.  BF16 rows (half as long) and
+// tensors with more modes (more operands than the split serves) measured
+// faster on the plain order with rank-ordered LDS rows of every operand read
+// through generic loads.
 static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int elem_bytes) {
     if (c->N < 3) return DYN_OK;
-    if (elem_bytes != 4) {
-        // BF16 operands run on the plain (row, secondary) order: undo a split
+    if (elem_bytes != 4 || c->N > 3) {
+        // plain (row, secondary) order: undo a split
         const ModeCopy& cp = c->copy[mode];
         return cp.hot_w >= 0 ? remap_mode(c, mode, cp.secondary) : DYN_OK;
     }

@@ -114,7 +114,7 @@ int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
  * outputs: rowptr_out[dims[mode]+1], perm_out[nnz] (sorted position -> input
  * position).  `secondary` >= 0 additionally orders each row by
  * idx[secondary] (composite key); -1 keeps input order within rows.
- * FP32-operand spMTTKRPs on three or more modes later reorder the nonzeros
+ * FP32-operand spMTTKRPs on 3-mode tensors later reorder the nonzeros
  * inside each row (hot split: those whose next-mode factor row is held in
  * LDS first, both classes stable); the row pointer is unchanged. */
 int dyn_gpu_remap(dyn_gpu_ctx* ctx, int mode, int secondary,
