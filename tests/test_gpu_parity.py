@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from dynasor_plus_amd import GpuContext, Tensor
+from dynasor_plus_amd import GpuContext, Tensor, host
 from dynasor_plus_amd import _lib as L
 from tests._oracle import tensor_from_npz
 from tests.conftest import golden
@@ -87,6 +87,46 @@ def test_mttkrp_vs_oracle_skewed(gpu_ctx, oracle, dims, nnz, R):
         want = oracle.mttkrp_f64(T, Y, n)
         got = gpu_ctx.mttkrp(n)
         assert frob_rel(got, want) <= 1e-5, (n, frob_rel(got, want))
+
+
+def test_mttkrp_hot_split_and_precision_switches(gpu_ctx, oracle):
+    """3-mode FP32 launches reorder each row hot-first by the next mode's rank
+    (rows beyond the LDS capacity stay in memory); BF16 launches restore the
+    plain order.  Operand modes larger than the LDS rows (5000 at rank 32)
+    give hot and cold heavy pieces; Zipf marginals give light rows too.
+    Alternating precisions on one context must stay exact, and every copy
+    must remain a row-grouped permutation of the input."""
+    say("mttkrp hot split")
+    T, _, _ = host.generate((300, 4000, 5000), 400_000, seed=17, coords="zipf", alpha=1.1,
+                            values="lognormal")
+    R = 32
+    rng = np.random.default_rng(17)
+    Y = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
+
+    def to_bf16(a):
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        return (u.astype(np.uint32) << 16).view(np.float32)
+
+    want32 = [oracle.mttkrp_f64(T, Y, n) for n in range(3)]
+    want16 = [oracle.mttkrp_f64(T, [to_bf16(y) for y in Y], n) for n in range(3)]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, R)
+    for prec in (L.DYN_PREC_FP32, L.DYN_PREC_BF16, L.DYN_PREC_FP32):
+        for n in range(3):
+            got = gpu_ctx.mttkrp(n, prec)
+            want = want32[n] if prec == L.DYN_PREC_FP32 else want16[n]
+            assert frob_rel(got, want) <= 1e-5, (prec, n, frob_rel(got, want))
+    # after the FP32 launches each copy is split; it still holds the input's
+    # (other coordinates, value) multiset
+    for n in range(3):
+        vals, idx = gpu_ctx.get_copy(n)
+        others = [w for w in range(3) if w != n]
+        got = np.lexsort([vals.view(np.uint32)] + [idx[w] for w in others][::-1])
+        ref = np.lexsort([T.vals.view(np.uint32)] + [T.idx[w] for w in others][::-1])
+        for w in others:
+            np.testing.assert_array_equal(idx[w][got], T.idx[w][ref])
+        np.testing.assert_array_equal(vals[got], T.vals[ref])
 
 
 def test_mttkrp_bf16_operands(gpu_ctx, oracle):
