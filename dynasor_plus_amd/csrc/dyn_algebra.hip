@@ -313,15 +313,21 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
 template <int RP>
 __global__ __launch_bounds__(kBlock) void k_gram_reduce(const float* __restrict__ gpart,
                                                         uint32_t nwg, double* __restrict__ G) {
-    __shared__ double red[4][64];
-    const int le = threadIdx.x & 63, sub = threadIdx.x >> 6;
-    const int e = blockIdx.x * 64 + le;
+    // 16 elements per block, 16 strided subsets of the partials per element,
+    // subsets combined in order
+    __shared__ double red[16][16];
+    const int le = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    const int e = blockIdx.x * 16 + le;
     double s = 0.0;
     if (e < RP * RP)
-        for (uint32_t b = sub; b < nwg; b += 4) s += (double)gpart[(size_t)b * RP * RP + e];
+        for (uint32_t b = sub; b < nwg; b += 16) s += (double)gpart[(size_t)b * RP * RP + e];
     red[sub][le] = s;
     __syncthreads();
-    if (sub == 0 && e < RP * RP) G[e] = ((red[0][le] + red[1][le]) + red[2][le]) + red[3][le];
+    if (sub == 0 && e < RP * RP) {
+        double t = 0.0;
+        for (int j = 0; j < 16; ++j) t += red[j][le];
+        G[e] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -433,28 +439,29 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve(const float* __restrict__
                                                        const double* __restrict__ Wg,
                                                        uint32_t rows,
                                                        double* __restrict__ sq_part) {
+    // four lanes per row, lane q of a row owning columns [q CL, (q + 1) CL)
+    constexpr int CL = RP / 4;
     __shared__ double W[RP * RP];
     __shared__ double red[kSolveBlock / 64][RP];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const int q = lane & 3;
+    const int g = lane >> 2;          // row of the wave's 16
     for (int e = tid; e < RP * RP; e += kSolveBlock) W[e] = Wg[e];
     __syncthreads();
-    constexpr int NA = (RP + 63) / 64;
-    double sq[NA];
+    double sq[CL];
 #pragma unroll
-    for (int j = 0; j < NA; ++j) sq[j] = 0.0;
+    for (int c = 0; c < CL; ++c) sq[c] = 0.0;
 
-    const uint32_t stride = gridDim.x * kSolveBlock;
-    // every thread of a wave runs the same number of trips (wave-uniform
-    // bound) so the cross-lane reductions below see all 64 lanes
-    const uint32_t base0 = blockIdx.x * kSolveBlock + (uint32_t)(wave * 64);
-    for (uint32_t b = base0; b < rows; b += stride) {
-        const uint32_t i = b + lane;
+    // wave-uniform trip count, so the cross-lane reductions see all lanes
+    const uint32_t stride = gridDim.x * (kSolveBlock / 4);
+    for (uint32_t b = blockIdx.x * (kSolveBlock / 4) + (uint32_t)(wave * 16); b < rows; b += stride) {
+        const uint32_t i = b + (uint32_t)g;
         const bool ok = i < rows;
-        double x[RP];
+        double x[CL];
 #pragma unroll
-        for (int c = 0; c < RP; ++c) x[c] = 0.0;
+        for (int c = 0; c < CL; ++c) x[c] = 0.0;
         const float* yrow = Yhat + (size_t)(ok ? i : 0) * RP;
         for (int r4 = 0; r4 < RP / 4; ++r4) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -462,32 +469,33 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve(const float* __restrict__
             const double yv[4] = {(double)v.x, (double)v.y, (double)v.z, (double)v.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const double* wr = W + (r4 * 4 + k) * RP;
+                const double* wr = W + (r4 * 4 + k) * RP + q * CL;
 #pragma unroll
-                for (int c = 0; c < RP; ++c) x[c] = fma(yv[k], wr[c], x[c]);
+                for (int c = 0; c < CL; ++c) x[c] = fma(yv[k], wr[c], x[c]);
             }
         }
         if (ok) {
 #pragma unroll
-            for (int c4 = 0; c4 < RP / 4; ++c4) {
+            for (int c4 = 0; c4 < CL / 4; ++c4) {
                 float4 v = make_float4((float)x[c4 * 4 + 0], (float)x[c4 * 4 + 1],
                                        (float)x[c4 * 4 + 2], (float)x[c4 * 4 + 3]);
-                *reinterpret_cast<float4*>(X + (size_t)i * RP + c4 * 4) = v;
+                *reinterpret_cast<float4*>(X + (size_t)i * RP + q * CL + c4 * 4) = v;
             }
-        }
-        // column sums of x^2 across the wave; lane (c % 64) keeps column c
 #pragma unroll
-        for (int c = 0; c < RP; ++c) {
-            double t = ok ? x[c] * x[c] : 0.0;
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
-            if (lane == (c & 63)) sq[c >> 6] += t;
+            for (int c = 0; c < CL; ++c) sq[c] = fma(x[c], x[c], sq[c]);
         }
     }
+    // column sums of x^2: over the 16 rows of the wave (lanes q, q+4, ...)
 #pragma unroll
-    for (int j = 0; j < NA; ++j) {
-        const int c = j * 64 + lane;
-        if (c < RP) red[wave][c] = sq[j];
+    for (int c = 0; c < CL; ++c) {
+        double t = sq[c];
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1) t += __shfl_xor(t, o, 64);
+        sq[c] = t;
+    }
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < CL; ++c) red[wave][q * CL + c] = sq[c];
     }
     __syncthreads();
     if (tid < RP) {
@@ -605,7 +613,7 @@ static int scale_gram_rp(dyn_gpu_ctx* c, int mode, const SolveParts* sq, bool in
     a.gpart = (float*)((char*)sp + (size_t)a.nwg * RP * sizeof(double));
     hipLaunchKernelGGL(k_scale_gram<RP>, dim3(a.nwg), dim3(kBlock), 0, c->stream, a);
     DYN_CHECK(check_launch(c, "scale_gram"));
-    hipLaunchKernelGGL(k_gram_reduce<RP>, dim3((RP * RP + 63) / 64), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_gram_reduce<RP>, dim3((RP * RP + 15) / 16), dim3(kBlock), 0,
                        c->stream, a.gpart, a.nwg, c->G[mode]);
     DYN_CHECK(check_launch(c, "gram_reduce"));
     if (ipart_out) *ipart_out = a.ipart;
@@ -650,7 +658,7 @@ static int chol(dyn_gpu_ctx* c, int n, double ridge, hipStream_t stream) {
 static int solve(dyn_gpu_ctx* c, int n, SolveParts* out) {
     const uint32_t rows = c->dims[n];
     const uint32_t nwg = std::max<uint32_t>(
-        1, std::min<uint32_t>((rows + kSolveBlock - 1) / kSolveBlock, kMaxSolveWG));
+        1, std::min<uint32_t>((rows + kSolveBlock / 4 - 1) / (kSolveBlock / 4), kMaxSolveWG));
     void* sp = nullptr;
     DYN_CHECK(scratch(c, (size_t)nwg * c->Rp * sizeof(double), &sp));
     double* part = (double*)sp;
