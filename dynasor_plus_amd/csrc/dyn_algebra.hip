@@ -82,6 +82,24 @@ int tensor_norm2(dyn_gpu_ctx* c, double* out) {
     return DYN_OK;
 }
 
+// sum over k of p[(first + k step) ld + col] for first + k step < n, added in
+// k order; the loads are issued eight at a time so that their latencies overlap
+template <typename T>
+__device__ __forceinline__ double strided_sum(const T* __restrict__ p, uint32_t first,
+                                              uint32_t step, uint32_t n, size_t ld, int col) {
+    double t = 0.0;
+    uint32_t b = first;
+    for (; b + 7u * step < n; b += 8u * step) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (double)p[(size_t)(b + k * step) * ld + col];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t += v[k];
+    }
+    for (; b < n; b += step) t += (double)p[(size_t)b * ld + col];
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 //  Scale + Gram pass.  Rows are consumed in MFMA operand order: with the
 //  32x32x2 instruction lane l holds row (l / 32) of a row pair at column
@@ -126,9 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
     if (scale) {
         constexpr int NS = kBlock / RP;
         const int col = tid % RP, sub = tid / RP;
-        double t = 0.0;
-        for (uint32_t b = sub; b < a.sq_nwg; b += NS) t += a.sq_part[(size_t)b * RP + col];
-        s_red[tid] = t;
+        s_red[tid] = strided_sum(a.sq_part, (uint32_t)sub, (uint32_t)NS, a.sq_nwg, RP, col);
         __syncthreads();
         if (tid < RP) {
             double v = 0.0;
@@ -151,23 +167,39 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
         const int rsub = lane >> 4;
         const double inv = scale ? s_inv[col] : 1.0;
         double ip = 0.0;
-        for (uint32_t r0 = (blockIdx.x * W + wave) * 4u; r0 < a.rows; r0 += a.nwg * W * 4u) {
-            const uint32_t r = r0 + rsub;
-            float y = 0.f;
-            if (r < a.rows) {
-                y = a.Y[(size_t)r * RP + col];
-                if (scale) {
-                    y = (float)((double)y * inv);
-                    a.Y[(size_t)r * RP + col] = y;
-                }
-                if (a.Yg) {
-                    const size_t g = (size_t)a.relab[r] * RP + col;
-                    a.Yg[g] = y;
-                    a.Ygb[g] = to_bf16(y);
-                }
-                if (a.Yhat) ip += (double)y * (double)a.Yhat[(size_t)r * RP + col];
+        const uint32_t stride = a.nwg * W * 4u;
+        // U row quads per trip: all loads first, then the updates in row order
+        constexpr int U = 4;
+        for (uint32_t r0 = (blockIdx.x * W + wave) * 4u; r0 < a.rows; r0 += stride * U) {
+            float x[U], yh[U];
+            uint32_t rl[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t r = r0 + u * stride + rsub;
+                const uint32_t rc = r < a.rows ? r : a.rows - 1;
+                x[u] = a.Y[(size_t)rc * RP + col];
+                yh[u] = a.Yhat ? a.Yhat[(size_t)rc * RP + col] : 0.f;
+                rl[u] = a.Yg ? a.relab[rc] : 0u;
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y, y, acc, 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t r = r0 + u * stride + rsub;
+                const bool ok = r < a.rows;
+                float y = x[u];
+                if (scale) y = (float)((double)y * inv);
+                if (ok) {
+                    if (scale) a.Y[(size_t)r * RP + col] = y;
+                    if (a.Yg) {
+                        const size_t g = (size_t)rl[u] * RP + col;
+                        a.Yg[g] = y;
+                        a.Ygb[g] = to_bf16(y);
+                    }
+                    if (a.Yhat) ip += (double)y * (double)yh[u];
+                } else {
+                    y = 0.f;
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y, y, acc, 0, 0, 0);
+            }
         }
         // deterministic cross-wave sum through LDS (wave order)
         for (int w = 0; w < W; ++w) {
@@ -205,19 +237,35 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
         double ip[TB];
 #pragma unroll
         for (int b = 0; b < TB; ++b) ip[b] = 0.0;
-        const uint32_t step = kSplitRows ? (uint32_t)W * 2u : 2u;
         const uint32_t first = kSplitRows ? (blockIdx.x * W + wave) * 2u : blockIdx.x * 2u;
         const uint32_t stride = kSplitRows ? a.nwg * W * 2u : a.nwg * 2u;
-        (void)step;
-        for (uint32_t r0 = first; r0 < a.rows; r0 += stride) {
-            const uint32_t r = r0 + rsub;
+        // U row pairs per trip: all loads first, then the updates in row order
+        constexpr int U = 4;
+        for (uint32_t r00 = first; r00 < a.rows; r00 += stride * U) {
+          float xl[U][TB], yh[U][TB];
+          uint32_t rl[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+              const uint32_t r = r00 + u * stride + rsub;
+              const uint32_t rc = r < a.rows ? r : a.rows - 1;
+#pragma unroll
+              for (int b = 0; b < TB; ++b) {
+                  const size_t off = (size_t)rc * RP + b * 32 + col;
+                  xl[u][b] = a.Y[off];
+                  yh[u][b] = a.Yhat ? a.Yhat[off] : 0.f;
+              }
+              rl[u] = a.Yg ? a.relab[rc] : 0u;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t r = r00 + u * stride + rsub;
             float y[TB];
 #pragma unroll
             for (int b = 0; b < TB; ++b) {
                 y[b] = 0.f;
                 if (r < a.rows) {
                     const size_t off = (size_t)r * RP + b * 32 + col;
-                    float x = a.Y[off];
+                    float x = xl[u][b];
                     if (scale) x = (float)((double)x * inv[b]);
                     y[b] = x;
                     // in the block-row split every wave reads the whole row;
@@ -226,11 +274,11 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
                     if (writer) {
                         if (scale) a.Y[off] = x;
                         if (a.Yg) {
-                            const size_t g = (size_t)a.relab[r] * RP + b * 32 + col;
+                            const size_t g = (size_t)rl[u] * RP + b * 32 + col;
                             a.Yg[g] = x;
                             a.Ygb[g] = to_bf16(x);
                         }
-                        if (a.Yhat) ip[b] += (double)x * (double)a.Yhat[off];
+                        if (a.Yhat) ip[b] += (double)x * (double)yh[u][b];
                     }
                 }
             }
@@ -250,6 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_scale_gram(ScaleArgs a) {
                 for (int bn = 0; bn < TB; ++bn)
                     acc[bn] = __builtin_amdgcn_mfma_f32_32x32x2f32(ym, y[bn], acc[bn], 0, 0, 0);
             }
+          }
         }
         // accumulator layout (32x32): register v of lane l holds
         // C[8*(v/4) + 4*(l/32) + v%4][l%32]
@@ -318,10 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(const float* __restrict_
     __shared__ double red[16][16];
     const int le = threadIdx.x & 15, sub = threadIdx.x >> 4;
     const int e = blockIdx.x * 16 + le;
-    double s = 0.0;
-    if (e < RP * RP)
-        for (uint32_t b = sub; b < nwg; b += 16) s += (double)gpart[(size_t)b * RP * RP + e];
-    red[sub][le] = s;
+    red[sub][le] = (e < RP * RP) ? strided_sum(gpart, (uint32_t)sub, 16u, nwg, (size_t)RP * RP, e)
+                                 : 0.0;
     __syncthreads();
     if (sub == 0 && e < RP * RP) {
         double t = 0.0;
@@ -519,33 +566,43 @@ struct FitArgs {
     double*       out;
 };
 
+// sum of one value per thread in a fixed order: butterfly within each wave,
+// then the waves in order; every thread gets the result
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+    return t;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
-    __shared__ double red[kBlock];
-    __shared__ double s_inner;
+    __shared__ double part[kBlock];
+    __shared__ double red[kBlock / 64];
     const int tid = threadIdx.x;
     // <T, M> = sum_r lambda_r * colsum_r; column sums over the partial blocks
     // in strided subsets, subsets combined in order
+    double inner;
     if (a.ipart) {
         const int NS = kBlock / a.RP;
         const int col = tid % a.RP, sub = tid / a.RP;
-        double t = 0.0;
-        if (sub < NS)
-            for (uint32_t b = sub; b < a.nwg; b += NS) t += a.ipart[(size_t)b * a.RP + col];
-        red[tid] = t;
+        part[tid] = (sub < NS) ? strided_sum(a.ipart, (uint32_t)sub, (uint32_t)NS, a.nwg,
+                                             (size_t)a.RP, col)
+                               : 0.0;
         __syncthreads();
-        if (tid == 0) {
-            double inner = 0.0;
-            for (int c = 0; c < a.R; ++c) {
-                double cs = 0.0;
-                for (int j = 0; j < NS; ++j) cs += red[j * a.RP + c];
-                inner += a.lambda[c] * cs;
-            }
-            s_inner = inner;
+        double v = 0.0;
+        if (tid < a.R) {
+            double cs = 0.0;
+            for (int j = 0; j < NS; ++j) cs += part[j * a.RP + tid];
+            v = a.lambda[tid] * cs;
         }
-    } else if (tid == 0) {
-        s_inner = *a.inner_dev;
+        inner = block_sum(v, red);
+    } else {
+        inner = *a.inner_dev;
     }
-    __syncthreads();
     // ||M||^2 = sum_{r,c} lambda_r lambda_c prod_n G_n[r, c]
     double rs = 0.0;
     for (int e = tid; e < a.R * a.R; e += kBlock) {
@@ -554,13 +611,9 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
         for (int n = 0; n < a.N; ++n) h *= a.G[n][(size_t)r * a.RP + c];
         rs += a.lambda[r] * (a.lambda[c] * h);
     }
-    red[tid] = rs;
-    __syncthreads();
+    double mn = block_sum(rs, red);
     if (tid == 0) {
-        double mn = 0.0;
-        for (int i = 0; i < kBlock; ++i) mn += red[i];
         mn = fmax(mn, 0.0);
-        const double inner = s_inner;
         double r2 = a.tnorm2 - 2.0 * inner + mn;
         if (r2 < 0.0) r2 = 0.0;
         const double resid = sqrt(r2);
@@ -789,7 +842,14 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         double mt = 0.0;
         for (int i = 0; i < nev; ++i) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, E[1 + 2 * i], E[2 + 2 * i]) == hipSuccess) mt += ms * 1e-3;
+            if (hipEventElapsedTime(&ms, E[1 + 2 * i], E[2 + 2 * i]) == hipSuccess) {
+                mt += ms * 1e-3;
+                // Gauss-Seidel pairs bracket exactly one segment kernel
+                if (c->timing && !jacobi) {
+                    c->timed_extra_ms += ms;
+                    c->timed_extra_n += 1;
+                }
+            }
         }
         return mt;
     };
@@ -823,9 +883,9 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                 break;
             }
             if (!jacobi) {
-                (void)hipEventRecord(E[1 + 2 * n], c->stream);
+                c->seg_pair = &E[1 + 2 * n];
                 rc = mttkrp_mode(c, n, o->precision, Yin, nullptr, c->Yhat[n], c->Rp);
-                (void)hipEventRecord(E[2 + 2 * n], c->stream);
+                c->seg_pair = nullptr;
                 nev = n + 1;
                 if (rc != DYN_OK) break;
             }

@@ -626,8 +626,10 @@ extern "C" int dyn_gpu_kernel_stats(dyn_gpu_ctx* c, uint64_t* launches, double* 
         tot += ms;
         c->event_pool.push_back(t);
     }
-    if (launches) *launches = c->timed.size();
-    if (total_ms) *total_ms = tot;
+    if (launches) *launches = c->timed.size() + c->timed_extra_n;
+    if (total_ms) *total_ms = tot + c->timed_extra_ms;
     c->timed.clear();
+    c->timed_extra_ms = 0.0;
+    c->timed_extra_n = 0;
     return DYN_OK;
 }

@@ -136,6 +136,12 @@ struct dyn_gpu_ctx {
     bool         timing = false;
     std::vector<dyn::TimedPair> timed;
     std::vector<dyn::TimedPair> event_pool;
+    // events of the CP-ALS loop that bracket one segment kernel in place of a
+    // timing pair (one record each side instead of two); the loop adds their
+    // times below when timing is on
+    const hipEvent_t* seg_pair = nullptr;
+    double       timed_extra_ms = 0.0;
+    uint64_t     timed_extra_n = 0;
 };
 
 namespace dyn {

@@ -825,15 +825,20 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
     }
     a.out = out;
     DYN_CHECK(fill_f32(c, out, (size_t)c->dims[mode] * Rp, 0.0f));
-    if (a.nseg == 0 && a.npieces == 0) return DYN_OK;
+    const hipEvent_t* ext = c->seg_pair;
     TimedPair tp;
-    timing_begin(c, &tp);
-    if (prec == DYN_PREC_BF16)
-        DYN_CHECK((dispatch_rp<bf16, 0>(c, Rp, NO, a, dims_o)));
-    else
-        DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a, dims_o)));
-    DYN_CHECK(check_launch(c, "mttkrp_seg"));
-    timing_end(c, &tp);
+    if (ext) (void)hipEventRecord(ext[0], c->stream);
+    else timing_begin(c, &tp);
+    if (a.nseg != 0 || a.npieces != 0) {
+        if (prec == DYN_PREC_BF16)
+            DYN_CHECK((dispatch_rp<bf16, 0>(c, Rp, NO, a, dims_o)));
+        else
+            DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a, dims_o)));
+        DYN_CHECK(check_launch(c, "mttkrp_seg"));
+    }
+    if (ext) (void)hipEventRecord(ext[1], c->stream);
+    else timing_end(c, &tp);
+    if (a.nseg == 0 && a.npieces == 0) return DYN_OK;
     return launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, out);
 }
 
