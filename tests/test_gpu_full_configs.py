@@ -14,9 +14,10 @@ do not need an oracle run over the whole tensor:
   * configuration 5 (1M x 100k x 2k, 200M uniform coordinates, Bernoulli
     labels from a planted rank-16 N(0, 0.5) model): 100 Adam steps at rank 32
     on the fused loss/adjoint, the final device loss against the FP64 oracle
-    loss, and device gradient entries against FP64 central differences of the
-    loss restricted to the nonzeros that the entry touches (the other terms
-    do not depend on it).
+    loss, and 10240 device gradient entries (every column of 320 seeded rows)
+    against FP64 central differences of the loss restricted to the nonzeros
+    that the entry touches (the other terms do not depend on it); the rows'
+    nonzeros come from the device remap's row pointers and permutation.
 """
 import sys
 
@@ -109,6 +110,10 @@ def test_c5_full_completion_adam_and_gradients():
                             planted_var=0.5, threads=16)
     orc = load_oracle()
     steps, R = 100, 32
+    rng = np.random.default_rng(7)
+    # the seeded gradient subset: 320 (mode, row) pairs, every column of each
+    picks = [(int(rng.integers(0, 3)), None) for _ in range(320)]
+    picks = [(n, int(rng.integers(0, T.dims[n]))) for n, _ in picks]
     with GpuContext(0) as eng:
         say("c5 full: upload")
         eng.upload(T)
@@ -117,34 +122,40 @@ def test_c5_full_completion_adam_and_gradients():
         final, grads = completion_loss_and_grad(eng, A, "bernoulli")
         A64 = [a.cpu().numpy().astype(np.float64) for a in A]
         G = [g.cpu().numpy() for g in grads]
+        say("c5 full: row segments of the picked rows")
+        segs = {}
+        for n in range(3):
+            rowptr, perm = eng.remap(n)
+            for m, i in picks:
+                if m == n:
+                    segs[(n, i)] = perm[int(rowptr[i]):int(rowptr[i + 1])].astype(np.int64)
     assert len(losses) == steps and np.all(np.isfinite(losses))
     assert losses[-1] < 0.95 * losses[0]
     say("c5 full: oracle loss")
     L64 = orc.completion_loss(T, A64, 1)
     assert abs(final - L64) / L64 < 1e-5, (final, L64)
-    say("c5 full: restricted central differences")
-    rng = np.random.default_rng(7)
+    say("c5 full: restricted central differences on 10240 gradient entries")
     h = 1e-4
-    fd, ad = [], []
     y = T.vals.astype(np.float64)
-    for _ in range(12):
-        n = int(rng.integers(0, 3))
-        i = int(rng.integers(0, T.dims[n]))
-        r = int(rng.integers(0, R))
-        k = np.nonzero(T.idx[n] == i)[0]
+    fd, ad = [], []
+    for n, i in picks:
+        k = segs[(n, i)]
+        assert np.all(T.idx[n][k] == i)
         if k.size == 0:
             continue
         rows = [A64[w][T.idx[w][k]] for w in range(3)]
+        m = (rows[0] * rows[1] * rows[2]).sum(axis=1)
+        others = [w for w in range(3) if w != n]
+        q = rows[others[0]] * rows[others[1]]          # dm / dA_n[i, r] per nonzero
+        yk = y[k]
 
-        def loss_at(delta):
-            rr = [x.copy() for x in rows]
-            rr[n][:, r] += delta
-            m = rr[0] * rr[1] * rr[2]
-            m = m.sum(axis=1)
-            return float(np.sum(np.logaddexp(0.0, m) - y[k] * m))
+        def loss_at(mm):
+            return float(np.sum(np.logaddexp(0.0, mm) - yk * mm))
 
-        fd.append((loss_at(h) - loss_at(-h)) / (2 * h))
-        ad.append(float(G[n][i, r]))
+        for r in range(R):
+            fd.append((loss_at(m + h * q[:, r]) - loss_at(m - h * q[:, r])) / (2 * h))
+            ad.append(float(G[n][i, r]))
     fd, ad = np.array(fd), np.array(ad)
-    assert fd.size >= 8
-    assert np.linalg.norm(ad - fd) / np.linalg.norm(fd) < 1e-4, (ad, fd)
+    say("fd entries", fd.size)
+    assert fd.size >= 9000
+    assert np.linalg.norm(ad - fd) / np.linalg.norm(fd) < 1e-4
