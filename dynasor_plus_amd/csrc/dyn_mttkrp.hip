@@ -36,6 +36,9 @@ namespace dyn {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kBufDword3 = 0x00020000;   // raw buffer resource word 3 on gfx9-family parts
+// cache policy of the stream loads: non-temporal (each value and index is
+// read once), so the stream does not push gathered factor rows out of L2
+constexpr int kStreamPolicy = 2;
 constexpr int kQuads = kBlock / 4;            // quads per 256-thread block
 
 struct SegArgs {
@@ -490,7 +493,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 const uint32_t boff = (kb + e0) * 4u;
 #pragma unroll
                 for (int w = 0; w < NO; ++w) {
-                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_idx[w], boff, 0, 0);
+                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_idx[w], boff, 0, kStreamPolicy);
 #pragma unroll
                     for (int c = 0; c < 4; ++c) si4[w][c] = (e0 + c < n) ? x[c] : 0u;
                 }
@@ -499,7 +502,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                     for (int c = 0; c < 4; ++c)
                         sv4[c] = (e0 + c < n) ? a.vals[a.vperm[kb + e0 + c]] : 0.f;
                 } else {
-                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_val, boff, 0, 0);
+                    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs_val, boff, 0, kStreamPolicy);
 #pragma unroll
                     for (int c = 0; c < 4; ++c) sv4[c] = (e0 + c < n) ? __uint_as_float(x[c]) : 0.f;
                 }
