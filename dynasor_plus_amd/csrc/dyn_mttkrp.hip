@@ -344,8 +344,9 @@ __device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)
     }
 }
 
-// Gathers of HB broadcast nonzeros (operand 0 reused from raw0prev when the
-// index repeats), then their accumulation in order.  `live(j)` says whether
+// Gathers of HB broadcast nonzeros (FP32: operand 0 reused from raw0prev when
+// the index repeats; BF16 rows are half as long and measured faster without
+// the reuse selects), then their accumulation in order.  `live(j)` says whether
 // nonzero j of the batch exists; `before_step(j)` runs ahead of its
 // accumulation (row changes of light segments).
 template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, int SPLIT = 0,
@@ -364,7 +365,7 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT
         for (int h = 0; h < HB; ++h) {
             const int j = j0 + h;
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
-            reuse[h] = (ib[j][0] == before);
+            reuse[h] = sizeof(FT) == 4 && ib[j][0] == before;
             if (!reuse[h])
                 gather_op<RP, FT, SPLIT>(a, hot1, 0, gb.g[0], gb.l[0], ib[j][0], raw[h][0]);
 #pragma unroll
