@@ -164,6 +164,7 @@ void free_copy(dyn_gpu_ctx* c, int m) {
     dfree(c, cp.heavy_row);
     dfree(c, cp.heavy_first);
     dfree(c, cp.heavy_count);
+    dfree(c, cp.empty_rows);
     cp = ModeCopy();
 }
 

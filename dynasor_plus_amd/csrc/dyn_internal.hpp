@@ -41,7 +41,9 @@ inline int padded_rank(int R) {
 //     (heavy_first / heavy_count) and are summed in slot order;
 //   * light rows are never split: consecutive light rows whose first nonzero
 //     falls in the same kLightWindow-aligned window form one segment
-//     [seg_k[i], seg_e[i]) starting at row seg_row[i], stored directly.
+//     [seg_k[i], seg_e[i]) starting at row seg_row[i], stored directly;
+//   * rows without nonzeros are listed in empty_rows and zeroed by the
+//     spMTTKRP kernel itself, so no output row needs clearing beforehand.
 struct ModeCopy {
     bool      built = false;
     int       secondary = -1;
@@ -67,6 +69,8 @@ struct ModeCopy {
     uint32_t* heavy_row = nullptr;              // nheavy
     uint32_t* heavy_first = nullptr;            // nheavy : first piece
     uint32_t* heavy_count = nullptr;            // nheavy : pieces
+    uint32_t  nempty = 0;
+    uint32_t* empty_rows = nullptr;             // nempty
 };
 
 constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a piece

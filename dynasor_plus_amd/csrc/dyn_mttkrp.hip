@@ -62,6 +62,8 @@ struct SegArgs {
     uint32_t        nwaves;       // waves of the launch (set by the launcher)
     uint32_t        stream_bytes; // bytes of one value / index column (buffer range)
     const uint32_t* pc_hot;       // hot split of operand 1 (null: none)
+    const uint32_t* empty_rows;   // rows without nonzeros, zeroed by the kernel
+    uint32_t        nempty;
     // completion kernel only
     const float*    An;           // output-mode factor rows (FP32, input order)
     double*         loss_part;    // [npieces + nseg] or null
@@ -449,6 +451,13 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
     const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane(
         (int)(blockIdx.x * (uint32_t)(TB / 64) + (uint32_t)(tid >> 6)));
 
+    // rows without nonzeros belong to no piece or segment: one wave zeroes
+    // each (RP / 4 lanes, 16 bytes each)
+    for (uint32_t e = gw; e < a.nempty; e += nwaves)
+        if (lane < RP / 4)
+            reinterpret_cast<float4*>(a.out + (size_t)a.empty_rows[e] * RP)[lane] =
+                make_float4(0.f, 0.f, 0.f, 0.f);
+
     // ---- phase 1: heavy-row pieces, one wave at a time ----
     // Wave gw owns pieces [range_begin(gw), range_begin(gw + 1)); its running row
     // sum is flushed into the slot of the last piece of each row it touches,
@@ -802,6 +811,8 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     a->pc_row = cp.pc_row;
     a->pc_hot = cp.pc_hot;
     a->npieces = cp.npieces;
+    a->empty_rows = cp.empty_rows;
+    a->nempty = cp.nempty;
     a->stream_bytes = (uint32_t)std::min<uint64_t>(c->nnz * 4ull, 0xFFFFFFF0ull);
     void* sp = nullptr;
     const size_t pieces = std::max<uint32_t>(cp.npieces, 1);
@@ -828,7 +839,9 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
         a.vperm = c->copy[mode].perm;
     }
     a.out = out;
-    DYN_CHECK(fill_f32(c, out, (size_t)c->dims[mode] * Rp, 0.0f));
+    // the kernel writes every row; with no nonzeros at all it is not launched
+    if (a.nseg == 0 && a.npieces == 0)
+        DYN_CHECK(fill_f32(c, out, (size_t)c->dims[mode] * Rp, 0.0f));
     const hipEvent_t* ext = c->seg_pair;
     TimedPair tp;
     if (ext) (void)hipEventRecord(ext[0], c->stream);
@@ -1044,8 +1057,10 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         a.out = grad[mode];
         a.An = A[mode];
         a.loss = loss;
-        DYN_CHECK(fill_f32(c, grad[mode], (size_t)c->dims[mode] * Rp, 0.0f));
-        if (a.nseg == 0 && a.npieces == 0) continue;
+        if (a.nseg == 0 && a.npieces == 0) {
+            DYN_CHECK(fill_f32(c, grad[mode], (size_t)c->dims[mode] * Rp, 0.0f));
+            continue;
+        }
         DYN_CHECK((dispatch_rp<float, 1>(c, Rp, NO, a, dims_o)));
         DYN_CHECK(check_launch(c, "completion_seg"));
         DYN_CHECK(launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, grad[mode]));

@@ -373,7 +373,7 @@ int ensure_gather(dyn_gpu_ctx* c) {
 static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32_t>& rp,
                           const std::vector<uint32_t>* nh = nullptr) {
     const uint32_t dim = (uint32_t)rp.size() - 1;
-    std::vector<uint32_t> sk, se, sr, pk, pe, pr, ph, hr, hf, hc;
+    std::vector<uint32_t> sk, se, sr, pk, pe, pr, ph, hr, hf, hc, er;
     bool open = false;
     uint32_t window = 0;
     auto cut = [&](uint32_t r, uint32_t b, uint32_t e, uint32_t hot) {
@@ -386,7 +386,10 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
     };
     for (uint32_t r = 0; r < dim; ++r) {
         const uint32_t L = rp[r + 1] - rp[r];
-        if (L == 0) continue;
+        if (L == 0) {
+            er.push_back(r);
+            continue;
+        }
         if (L > kHeavyMin) {
             if (open) {
                 se.push_back(rp[r]);
@@ -415,13 +418,15 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
     }
     if (open) se.push_back(rp[dim]);
     for (uint32_t** t : {&cp.seg_k, &cp.seg_e, &cp.seg_row, &cp.pc_k, &cp.pc_e, &cp.pc_row,
-                         &cp.pc_hot, &cp.heavy_row, &cp.heavy_first, &cp.heavy_count}) {
+                         &cp.pc_hot, &cp.heavy_row, &cp.heavy_first, &cp.heavy_count,
+                         &cp.empty_rows}) {
         dfree(c, *t);
         *t = nullptr;
     }
     cp.nseg = (uint32_t)sk.size();
     cp.npieces = (uint32_t)pk.size();
     cp.nheavy = (uint32_t)hr.size();
+    cp.nempty = (uint32_t)er.size();
     auto up = [&](uint32_t** d, const std::vector<uint32_t>& v) -> int {
         DYN_CHECK(dalloc_t(c, d, std::max<size_t>(v.size(), 1)));
         if (!v.empty())
@@ -438,6 +443,7 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
     DYN_CHECK(up(&cp.heavy_row, hr));
     DYN_CHECK(up(&cp.heavy_first, hf));
     DYN_CHECK(up(&cp.heavy_count, hc));
+    DYN_CHECK(up(&cp.empty_rows, er));
     DYN_HIP(c, hipStreamSynchronize(c->stream));
     return DYN_OK;
 }

@@ -155,3 +155,51 @@ def test_adam_completion_driver(problem):
     L64 = orc.completion_loss(problem, [a.cpu().numpy().astype(np.float64) for a in A], 1)
     assert abs(final - L64) / L64 < 1e-5
     assert final < losses[-1]
+
+
+def test_empty_rows_zeroed_in_caller_buffers():
+    """Rows without nonzeros are written (as zeros) by the kernels themselves:
+    the caller's NaN-filled output buffers come back finite, with exact zeros
+    on the empty rows and the FP64 oracle elsewhere."""
+    say("empty rows in caller buffers")
+    import ctypes as C
+    from dynasor_plus_amd import _lib as L
+    from dynasor_plus_amd.autograd import _ptrs
+    orc = load_oracle()
+    rng = np.random.default_rng(9)
+    dims, nnz, R = (40, 30, 20), 3000, 16
+    idx = []
+    for d in dims:
+        allowed = np.array([r for r in range(d) if r % 4 != 1])     # every fourth row empty
+        idx.append(rng.choice(allowed, nnz).astype(np.uint32))
+    idx[0][:700] = 2                                                 # one heavy row
+    T = Tensor(dims, rng.standard_normal(nnz).astype(np.float32), idx)
+    A = [(rng.standard_normal((d, R)) * 0.5).astype(np.float32) for d in dims]
+    g = rng.standard_normal(nnz).astype(np.float32)
+    with GpuContext(0) as eng:
+        eng.upload(T)
+        At = [torch.tensor(a, device="cuda") for a in A]
+        gd = torch.tensor(g, device="cuda")
+        adj = [torch.full((d, R), float("nan"), device="cuda") for d in dims]
+        comp = [torch.full((d, R), float("nan"), device="cuda") for d in dims]
+        loss = C.c_double(0.0)
+        with eng.on_stream(torch.cuda.current_stream().cuda_stream):
+            for n in range(3):
+                eng._check(eng.lib.dyn_gpu_adjoint_dev(eng.h, L._vp(gd.data_ptr()), n, _ptrs(At), R,
+                                                       L._vp(adj[n].data_ptr())), "adjoint_dev")
+            eng._check(eng.lib.dyn_gpu_completion_grad_dev(eng.h, L.DYN_LOSS_LS, _ptrs(At), R,
+                                                           _ptrs(comp), C.byref(loss)),
+                       "completion_grad_dev")
+        torch.cuda.synchronize()
+        adj = [x.cpu().numpy() for x in adj]
+        comp = [x.cpu().numpy() for x in comp]
+    resid = orc.forward(T, A) - T.vals.astype(np.float64)
+    for n, d in enumerate(dims):
+        empty = np.setdiff1d(np.arange(d), idx[n])
+        assert empty.size >= d // 4
+        for got in (adj[n], comp[n]):
+            assert np.all(np.isfinite(got))
+            assert not got[empty].any()
+        assert frob_rel(adj[n], orc.mttkrp_f64(Tensor(dims, g, idx), A, n)) < 1e-5
+        assert frob_rel(comp[n], orc.mttkrp_f64(Tensor(dims, resid.astype(np.float32), idx),
+                                                A, n)) < 1e-4
