@@ -368,8 +368,10 @@ int ensure_gather(dyn_gpu_ctx* c) {
 // Work tables from the row pointer (host): heavy rows cut into kPiece
 // pieces, light rows grouped by the kLightWindow window of their first
 // nonzero.  With per-row hot counts `nh` (a hot split), a heavy row's pieces
-// are cut separately in its hot prefix and its cold rest, and each piece
-// records which class it holds.
+// are cut separately in its hot prefix and its cold rest, each piece records
+// which class it holds, and the two classes are interleaved in proportion
+// within the row so that every wave's contiguous range of pieces gets about
+// the same mix of fast (LDS) and slow (memory) pieces.
 static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32_t>& rp,
                           const std::vector<uint32_t>* nh = nullptr) {
     const uint32_t dim = (uint32_t)rp.size() - 1;
@@ -399,8 +401,20 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
             hf.push_back((uint32_t)pk.size());
             if (nh) {
                 const uint32_t split = rp[r] + (*nh)[r];
-                cut(r, rp[r], split, 1u);
-                cut(r, split, rp[r + 1], 0u);
+                const uint32_t nH = (split - rp[r] + kPiece - 1) / kPiece;
+                const uint32_t nC = (rp[r + 1] - split + kPiece - 1) / kPiece;
+                const uint64_t tot = (uint64_t)nH + nC;
+                uint32_t ih = 0, ic = 0;
+                for (uint64_t t = 0; t < tot; ++t) {
+                    const bool hot = ic >= nC || (ih < nH && (uint64_t)ih * tot < (t + 1) * nH);
+                    const uint32_t b = hot ? rp[r] + ih * kPiece : split + ic * kPiece;
+                    const uint32_t e = hot ? std::min(b + kPiece, split) : std::min(b + kPiece, rp[r + 1]);
+                    pk.push_back(b);
+                    pe.push_back(e);
+                    pr.push_back(r);
+                    ph.push_back(hot ? 1u : 0u);
+                    if (hot) ++ih; else ++ic;
+                }
             } else {
                 cut(r, rp[r], rp[r + 1], 0u);
             }
