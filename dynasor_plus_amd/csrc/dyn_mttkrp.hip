@@ -446,10 +446,12 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
 
     // Static schedule (pieces have equal length but the last of each row, so
     // round-robin balances them; a shared work counter would serialise on one
-    // L2 atomic per claim).
+    // L2 atomic per claim).  Wave numbers are strided over the workgroups, so
+    // the ranges one CU runs come from across the whole piece list and every
+    // CU gets about the same mix of rows.
     const uint32_t nwaves = gridDim.x * (uint32_t)(TB / 64);
     const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane(
-        (int)(blockIdx.x * (uint32_t)(TB / 64) + (uint32_t)(tid >> 6)));
+        (int)((uint32_t)(tid >> 6) * gridDim.x + blockIdx.x));
 
     // rows without nonzeros belong to no piece or segment: one wave zeroes
     // each (RP / 4 lanes, 16 bytes each)
