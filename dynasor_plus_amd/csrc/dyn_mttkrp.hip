@@ -284,9 +284,10 @@ struct GatherBase {
     const FT* l[NO];
 };
 
-// Whether heavy pieces take the hot-split path (FP32 operands, N = 3).
+// Whether heavy pieces take the hot-split path (3-mode tensors, both
+// operand precisions).
 template <int NO, typename FT>
-constexpr bool kSplitPath() { return NO == 2 && sizeof(FT) == 4; }
+constexpr bool kSplitPath() { return NO == 2; }
 
 // Gathers of operand w.  SPLIT 0: no hot split (LDS or memory by rank).
 // SPLIT 1: a hot-split piece -- operand 1 comes from LDS when the piece is
@@ -542,7 +543,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 do_rounds(std::integral_constant<int, 2>{}, splitc);
                 if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{}, splitc);
             };
-            // FP32 launches on three modes run on a hot-split copy
+            // launches on three modes run on a hot-split copy
             do_piece(std::integral_constant<int, kSplitPath<NO, FT>() ? 1 : 0>{});
             const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
             const bool flush = row_ends || (p + 1 == p1);
@@ -686,21 +687,23 @@ __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part
 // ---------------------------------------------------------------------------
 //  host-side dispatch
 // ---------------------------------------------------------------------------
-// Rows of operand 1 held in LDS under a hot split (the whole budget).
-static uint32_t hot_split_rows(dyn_gpu_ctx* c, int Rp, int elem_bytes, uint32_t rows) {
+// Rows of operand 1 held in LDS under a hot split: as many FP32 rows as the
+// budget holds, for either precision, so FP32 and BF16 launches share one
+// split copy.  BF16 rows take half the budget; the rest holds operand 0's
+// leading rows.
+static uint32_t hot_split_rows(dyn_gpu_ctx* c, int Rp, uint32_t rows) {
     const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
-    return (uint32_t)std::min<size_t>(rows, budget / ((size_t)Rp * elem_bytes));
+    return (uint32_t)std::min<size_t>(rows, budget / ((size_t)Rp * 4));
 }
 
-// The hot split used by FP32-operand spMTTKRP launches on 3-mode tensors:
-// operand 1 (the mode after the secondary sort mode) is split at the rows
-// that fit in LDS for this rank padding.  BF16 rows (half as long) and
-// tensors with more modes (more operands than the split serves) measured
-// faster on the plain order with rank-ordered LDS rows of every operand read
-// through generic loads.
-static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int elem_bytes) {
+// The hot split used by spMTTKRP launches on 3-mode tensors: operand 1 (the
+// mode after the secondary sort mode) is split at hot_split_rows.  Tensors
+// with more modes (more operands than the split serves) measured faster on
+// the plain order with rank-ordered LDS rows of every operand read through
+// generic loads.
+static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp) {
     if (c->N < 3) return DYN_OK;
-    if (elem_bytes != 4 || c->N > 3) {
+    if (c->N > 3) {
         // plain (row, secondary) order: undo a split
         const ModeCopy& cp = c->copy[mode];
         return cp.hot_w >= 0 ? remap_mode(c, mode, cp.secondary) : DYN_OK;
@@ -709,7 +712,7 @@ static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int elem_bytes) {
     for (int w = 0; w < c->N; ++w)
         if (w != mode) others[j++] = w;
     return ensure_hot_split(c, mode, others[1],
-                            hot_split_rows(c, Rp, elem_bytes, c->dims[others[1]]));
+                            hot_split_rows(c, Rp, c->dims[others[1]]));
 }
 
 template <int RP, int NO, typename FT, int KIND>
@@ -722,7 +725,7 @@ static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
         // hot split: operand 1's leading rows, exactly the split, then as many
         // of operand 0's as the rest of the budget holds
         for (int w = 0; w < NO; ++w) a.hot[w] = 0;
-        a.hot[1] = hot_split_rows(c, RP, (int)sizeof(FT), dims_o[1]);
+        a.hot[1] = hot_split_rows(c, RP, dims_o[1]);
         a.hot[0] = std::min<uint32_t>(
             dims_o[0], (uint32_t)((budget - std::min(budget, (size_t)a.hot[1] * rowb)) / rowb));
         bytes = ((size_t)a.hot[0] + a.hot[1]) * rowb;
@@ -834,7 +837,7 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
     int NO = 0;
     uint32_t dims_o[kMaxModes];
     if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
-    DYN_CHECK(prepare_split(c, mode, Rp, prec == DYN_PREC_BF16 ? 2 : 4));
+    DYN_CHECK(prepare_split(c, mode, Rp));
     DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false));
     if (vals_override) {
         a.vals = vals_override;
@@ -1054,7 +1057,7 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         uint32_t dims_o[kMaxModes];
         const bool want_loss = (mode == 0 && loss_out);
         if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
-        DYN_CHECK(prepare_split(c, mode, Rp, 4));
+        DYN_CHECK(prepare_split(c, mode, Rp));
         DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, want_loss));
         a.out = grad[mode];
         a.An = A[mode];

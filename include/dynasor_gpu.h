@@ -114,9 +114,9 @@ int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
  * outputs: rowptr_out[dims[mode]+1], perm_out[nnz] (sorted position -> input
  * position).  `secondary` >= 0 additionally orders each row by
  * idx[secondary] (composite key); -1 keeps input order within rows.
- * FP32-operand spMTTKRPs on 3-mode tensors later reorder the nonzeros
- * inside each row (hot split: those whose next-mode factor row is held in
- * LDS first, both classes stable); the row pointer is unchanged. */
+ * spMTTKRPs on 3-mode tensors (FP32 or BF16 operands) later reorder the
+ * nonzeros inside each row (hot split: those whose next-mode factor row is
+ * held in LDS first, both classes stable); the row pointer is unchanged. */
 int dyn_gpu_remap(dyn_gpu_ctx* ctx, int mode, int secondary,
                   uint64_t* rowptr_out, uint32_t* perm_out);
 /* Build every mode's copy (secondary key: the first other mode). */

@@ -90,12 +90,14 @@ def test_mttkrp_vs_oracle_skewed(gpu_ctx, oracle, dims, nnz, R):
 
 
 def test_mttkrp_hot_split_and_precision_switches(gpu_ctx, oracle):
-    """3-mode FP32 launches reorder each row hot-first by the next mode's rank
-    (rows beyond the LDS capacity stay in memory); BF16 launches restore the
-    plain order.  Operand modes larger than the LDS rows (5000 at rank 32)
-    give hot and cold heavy pieces; Zipf marginals give light rows too.
-    Alternating precisions on one context must stay exact, and every copy
-    must remain a row-grouped permutation of the input."""
+    """3-mode launches reorder each row hot-first by the next mode's rank
+    (rows beyond the LDS capacity stay in memory); FP32 and BF16 launches
+    share the split copy, BF16 ones with operand 0's leading rows in the LDS
+    half that BF16 rows leave free.  Operand modes larger than the LDS rows
+    (1272 at rank 32) give hot and cold heavy pieces and mixed LDS / memory
+    gathers of operand 0; Zipf marginals give light rows too.  Alternating
+    precisions on one context must stay exact, and every copy must remain a
+    row-grouped permutation of the input."""
     say("mttkrp hot split")
     T, _, _ = host.generate((300, 4000, 5000), 400_000, seed=17, coords="zipf", alpha=1.1,
                             values="lognormal")
