@@ -184,34 +184,35 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 }
 
 __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = i;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = (uint32_t)i;
 }
 
 __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm,
                              uint32_t* __restrict__ dst, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = src[perm[i]];
 }
 
 __global__ void k_gather_f32(const float* __restrict__ src, const uint32_t* __restrict__ perm,
                              float* __restrict__ dst, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = src[perm[i]];
 }
 
 // rowptr[r] = first sorted position whose key >= r, for r in [0, dim]
 __global__ void k_rowptr(const uint32_t* __restrict__ keys, uint32_t n, uint32_t dim,
                          uint32_t* __restrict__ rowptr) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
+    // 64-bit positions: a 32-bit one would wrap past 2^32 - stride
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; p <= n; p += stride) {
         const uint32_t lo = (p == 0) ? 0u : keys[p - 1] + 1u;   // rows (keys[p-1], ...]
         const uint32_t hi = (p == n) ? dim : keys[p];           // ... keys[p]]
-        for (uint32_t r = lo; r <= hi; ++r) rowptr[r] = p;
+        for (uint64_t r = lo; r <= hi; ++r) rowptr[r] = (uint32_t)p;
     }
 }
 
@@ -219,8 +220,8 @@ __global__ void k_rowptr(const uint32_t* __restrict__ keys, uint32_t n, uint32_t
 __global__ void k_gather_relabel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm,
                                  const uint32_t* __restrict__ relab, uint32_t* __restrict__ dst,
                                  uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = relab[src[perm[i]]];
 }
 
@@ -236,7 +237,8 @@ __global__ __launch_bounds__(kHistBlock) void k_bincount(const uint32_t* __restr
     extern __shared__ uint32_t hist[];
     for (uint32_t b = threadIdx.x; b < nb; b += kHistBlock) hist[b] = 0u;
     __syncthreads();
-    for (uint32_t i = blockIdx.x * kHistBlock + threadIdx.x; i < n; i += gridDim.x * kHistBlock) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kHistBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kHistBlock) {
         const uint32_t v = ix[i] - lo;
         if (v < nb) atomicAdd(&hist[v], 1u);
     }
@@ -467,16 +469,16 @@ static int build_segments(dyn_gpu_ctx* c, ModeCopy& cp, const std::vector<uint32
 // ---------------------------------------------------------------------------
 __global__ void k_hot_flag(const uint32_t* __restrict__ ix, uint32_t n, uint32_t H,
                            uint32_t* __restrict__ f) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += stride)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += stride)
         f[k] = (k < n && ix[k] < H) ? 1u : 0u;
 }
 
 // per row: hot count = hs[rp[r+1]] - hs[rp[r]] (hs: exclusive scan of the flags)
 __global__ void k_row_hot(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ hs,
                           uint32_t dim, uint32_t* __restrict__ nh) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += stride)
         nh[r] = hs[rp[r + 1]] - hs[rp[r]];
 }
 
@@ -497,8 +499,9 @@ struct HotScatter {
 };
 
 __global__ void k_hot_scatter(HotScatter a) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k64 < a.n; k64 += stride) {
+        const uint32_t k = (uint32_t)k64;
         // row of k: the last r with rp[r] <= k
         uint32_t lo = 0, hi = a.dim;
         while (hi - lo > 1) {
