@@ -82,23 +82,20 @@ int check_launch(dyn_gpu_ctx* c, const char* what) {
     return DYN_OK;
 }
 
-void timing_begin(dyn_gpu_ctx* c, TimedPair* tp) {
-    if (!c->timing) return;
+bool timing_acquire(dyn_gpu_ctx* c, TimedPair* tp) {
+    if (!c->timing) return false;
     if (c->event_pool.empty()) {
         TimedPair t;
-        (void)hipEventCreate(&t.a);
-        (void)hipEventCreate(&t.b);
+        if (hipEventCreate(&t.a) != hipSuccess) return false;
+        if (hipEventCreate(&t.b) != hipSuccess) {
+            (void)hipEventDestroy(t.a);
+            return false;
+        }
         c->event_pool.push_back(t);
     }
     *tp = c->event_pool.back();
     c->event_pool.pop_back();
-    (void)hipEventRecord(tp->a, c->stream);
-}
-
-void timing_end(dyn_gpu_ctx* c, TimedPair* tp) {
-    if (!c->timing) return;
-    (void)hipEventRecord(tp->b, c->stream);
-    c->timed.push_back(*tp);
+    return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -145,6 +145,9 @@ struct dyn_gpu_ctx {
     // timing pair (one record each side instead of two); the loop adds their
     // times below when timing is on
     const hipEvent_t* seg_pair = nullptr;
+    // events the next segment-kernel launch records itself, at its start
+    // and end (part of the dispatch, so no separate marker on the stream)
+    hipEvent_t   launch_ev[2] = {nullptr, nullptr};
     double       timed_extra_ms = 0.0;
     uint64_t     timed_extra_n = 0;
 };
@@ -220,7 +223,6 @@ int relabel_rows(dyn_gpu_ctx* c, const float* src, const uint32_t* relab, float*
 void free_relabel(dyn_gpu_ctx* c);
 int fill_f32(dyn_gpu_ctx* c, float* p, size_t n, float v);
 
-void timing_begin(dyn_gpu_ctx* c, TimedPair* tp);
-void timing_end(dyn_gpu_ctx* c, TimedPair* tp);
+bool timing_acquire(dyn_gpu_ctx* c, TimedPair* tp);   // a pool pair, not yet recorded
 
 }  // namespace dyn

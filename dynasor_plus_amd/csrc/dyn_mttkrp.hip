@@ -29,6 +29,8 @@
 //   No atomics on data: results are bitwise reproducible run to run.
 #include "dyn_internal.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -744,7 +746,13 @@ static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
         attr_set = true;
     }
     a.nwaves = (uint32_t)c->num_cus * (uint32_t)(seg_threads<RP, NO, KIND>() / 64);
-    hipLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()), bytes, c->stream, a);
+    if (c->launch_ev[0]) {
+        hipExtLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()),
+                              (uint32_t)bytes, c->stream, c->launch_ev[0], c->launch_ev[1], 0u, a);
+    } else {
+        hipLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()), bytes,
+                           c->stream, a);
+    }
     return DYN_OK;
 }
 
@@ -847,19 +855,32 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
     // the kernel writes every row; with no nonzeros at all it is not launched
     if (a.nseg == 0 && a.npieces == 0)
         DYN_CHECK(fill_f32(c, out, (size_t)c->dims[mode] * Rp, 0.0f));
-    const hipEvent_t* ext = c->seg_pair;
+    // timing events (the CP-ALS loop's pair, or a pool pair) are recorded by
+    // the launch itself
     TimedPair tp;
-    if (ext) (void)hipEventRecord(ext[0], c->stream);
-    else timing_begin(c, &tp);
-    if (a.nseg != 0 || a.npieces != 0) {
-        if (prec == DYN_PREC_BF16)
-            DYN_CHECK((dispatch_rp<bf16, 0>(c, Rp, NO, a, dims_o)));
-        else
-            DYN_CHECK((dispatch_rp<float, 0>(c, Rp, NO, a, dims_o)));
-        DYN_CHECK(check_launch(c, "mttkrp_seg"));
+    const bool pooled = !c->seg_pair && timing_acquire(c, &tp);
+    if (c->seg_pair) {
+        c->launch_ev[0] = c->seg_pair[0];
+        c->launch_ev[1] = c->seg_pair[1];
+    } else if (pooled) {
+        c->launch_ev[0] = tp.a;
+        c->launch_ev[1] = tp.b;
     }
-    if (ext) (void)hipEventRecord(ext[1], c->stream);
-    else timing_end(c, &tp);
+    int rc = DYN_OK;
+    if (a.nseg != 0 || a.npieces != 0) {
+        if (prec == DYN_PREC_BF16) rc = dispatch_rp<bf16, 0>(c, Rp, NO, a, dims_o);
+        else rc = dispatch_rp<float, 0>(c, Rp, NO, a, dims_o);
+        if (rc == DYN_OK) rc = check_launch(c, "mttkrp_seg");
+    } else if (c->launch_ev[0]) {
+        (void)hipEventRecord(c->launch_ev[0], c->stream);
+        (void)hipEventRecord(c->launch_ev[1], c->stream);
+    }
+    c->launch_ev[0] = c->launch_ev[1] = nullptr;
+    if (pooled) {
+        if (rc == DYN_OK) c->timed.push_back(tp);
+        else c->event_pool.push_back(tp);
+    }
+    DYN_CHECK(rc);
     if (a.nseg == 0 && a.npieces == 0) return DYN_OK;
     return launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, out);
 }

@@ -208,9 +208,10 @@ int dyn_gpu_completion_grad_dev(dyn_gpu_ctx* ctx, int loss, const float* const* 
                                 int ld, float* const* grad, double* loss_out);
 
 /* ---- timing ---------------------------------------------------------- */
-/* When enabled, every spMTTKRP launch is bracketed by HIP events on the
- * launch stream; dyn_gpu_kernel_stats returns (and resets) the number of
- * launches and their summed duration in milliseconds. */
+/* When enabled, every spMTTKRP kernel launch records a pair of HIP events
+ * at its start and end (as part of the dispatch, on the launch stream);
+ * dyn_gpu_kernel_stats returns (and resets) the number of launches and
+ * their summed duration in milliseconds. */
 int dyn_gpu_enable_timing(dyn_gpu_ctx* ctx, int on);
 int dyn_gpu_kernel_stats(dyn_gpu_ctx* ctx, uint64_t* launches, double* total_ms);
 
