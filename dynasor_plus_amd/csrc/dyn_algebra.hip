@@ -629,9 +629,9 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
 //  host wrappers
 // ---------------------------------------------------------------------------
 static uint32_t gram_wgs(uint32_t rows, int RP) {
-    // ~128 rows per workgroup: a few hundred workgroups for the largest
+    // ~64 rows per workgroup: several hundred workgroups for the largest
     // factors (the pass is latency-bound), and the partial Grams stay small
-    const uint32_t per = 128u;
+    const uint32_t per = 64u;
     uint32_t want = (rows + per - 1) / per;
     return std::max<uint32_t>(1, std::min<uint32_t>(want, kMaxGramWG));
 }
