@@ -203,3 +203,28 @@ def test_empty_rows_zeroed_in_caller_buffers():
         assert frob_rel(adj[n], orc.mttkrp_f64(Tensor(dims, g, idx), A, n)) < 1e-5
         assert frob_rel(comp[n], orc.mttkrp_f64(Tensor(dims, resid.astype(np.float32), idx),
                                                 A, n)) < 1e-4
+
+
+@pytest.mark.parametrize("dims", [(40, 30, 20, 10), (20, 16, 12, 10, 8)])
+def test_fused_ls_gradient_more_modes(dims):
+    """The fused loss/gradient kernel with three and four gathered operands:
+    least-squares gradients against the FP64 MTTKRP of the residual tensor,
+    and the loss against the oracle's."""
+    say("fused ls gradient", dims)
+    from dynasor_plus_amd.autograd import completion_loss_and_grad
+    orc = load_oracle()
+    rng = np.random.default_rng(len(dims))
+    T, _, _ = host.generate(dims, 6000, seed=len(dims) + 30, coords="uniform", values="lognormal")
+    R = 8
+    A = [(rng.standard_normal((d, R)) * 0.5).astype(np.float32) for d in dims]
+    with GpuContext(0) as eng:
+        eng.upload(T)
+        loss, grads = completion_loss_and_grad(eng, [torch.tensor(a, device="cuda") for a in A], "ls")
+        grads = [g.cpu().numpy() for g in grads]
+    m = orc.forward(T, A)
+    resid = m - T.vals.astype(np.float64)
+    want_loss = 0.5 * float(np.sum(resid * resid))
+    assert abs(loss - want_loss) / want_loss < 1e-5
+    Tr = Tensor(T.dims, resid.astype(np.float32), T.idx)
+    for n in range(len(dims)):
+        assert frob_rel(grads[n], orc.mttkrp_f64(Tr, A, n)) < 1e-4, n

@@ -75,7 +75,8 @@ def test_mttkrp_matches_reference_verify(gpu_ctx, oracle, name):
 @pytest.mark.parametrize("dims,nnz,R", [
     ((40, 30, 20), 5000, 5), ((64, 48, 32), 20000, 16), ((300, 200, 100), 60000, 32),
     ((50, 40, 30, 20), 30000, 64), ((30, 20, 16, 12, 8), 20000, 32), ((200, 150), 9000, 20),
-    ((20, 16, 12, 10, 8, 6), 15000, 8), ((100, 80, 60), 40000, 100)])
+    ((20, 16, 12, 10, 8, 6), 15000, 8), ((100, 80, 60), 40000, 100),
+    ((8, 6, 5, 4, 4, 3, 3), 12000, 16), ((6, 5, 5, 4, 4, 3, 3, 3), 20000, 16)])
 def test_mttkrp_vs_oracle_skewed(gpu_ctx, oracle, dims, nnz, R):
     say("mttkrp skewed", dims, nnz, R)
     rng = np.random.default_rng(nnz + R)
@@ -131,10 +132,11 @@ def test_mttkrp_hot_split_and_precision_switches(gpu_ctx, oracle):
         np.testing.assert_array_equal(vals[got], T.vals[ref])
 
 
-def test_mttkrp_bf16_operands(gpu_ctx, oracle):
-    say("mttkrp bf16")
+@pytest.mark.parametrize("dims", [(120, 90, 70), (60, 50, 40, 30), (30, 24, 20, 16, 12)])
+def test_mttkrp_bf16_operands(gpu_ctx, oracle, dims):
+    say("mttkrp bf16", dims)
     rng = np.random.default_rng(11)
-    T = skewed_tensor(rng, (120, 90, 70), 30000)
+    T = skewed_tensor(rng, dims, 30000)
     R = 32
     Y = [rng.standard_normal((d, R)).astype(np.float32) for d in T.dims]
     gpu_ctx.upload(T)
@@ -146,7 +148,7 @@ def test_mttkrp_bf16_operands(gpu_ctx, oracle):
         return (u.astype(np.uint32) << 16).view(np.float32)
 
     Yb = [to_bf16(y) for y in Y]
-    for n in range(3):
+    for n in range(len(dims)):
         got = gpu_ctx.mttkrp(n, L.DYN_PREC_BF16)
         assert frob_rel(got, oracle.mttkrp_f64(T, Yb, n)) <= 1e-5
         assert frob_rel(got, oracle.mttkrp_f64(T, Y, n)) <= 2e-2
