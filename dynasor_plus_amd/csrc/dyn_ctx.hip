@@ -82,6 +82,13 @@ int check_launch(dyn_gpu_ctx* c, const char* what) {
     return DYN_OK;
 }
 
+void lds_limit_once(dyn_gpu_ctx* c, std::atomic<uint64_t>* done, const void* fn, int bytes) {
+    const uint64_t bit = (c->device >= 0 && c->device < 64) ? (1ull << c->device) : 0ull;
+    if (bit && (done->load(std::memory_order_acquire) & bit)) return;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (bit) done->fetch_or(bit, std::memory_order_acq_rel);
+}
+
 bool timing_acquire(dyn_gpu_ctx* c, TimedPair* tp) {
     if (!c->timing) return false;
     if (c->event_pool.empty()) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -184,6 +185,10 @@ int scratch(dyn_gpu_ctx* c, size_t bytes, void** p);
 int scratch2(dyn_gpu_ctx* c, size_t bytes, void** p);
 
 int check_launch(dyn_gpu_ctx* c, const char* what);
+// Raise a kernel's dynamic-LDS limit once per device (the attribute is set
+// for the current device); `done` is the kernel's own flag word, one bit per
+// device id below 64.
+void lds_limit_once(dyn_gpu_ctx* c, std::atomic<uint64_t>* done, const void* fn, int bytes);
 
 // building blocks implemented in the .hip units -----------------------------
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary);

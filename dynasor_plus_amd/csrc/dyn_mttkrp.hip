@@ -739,12 +739,8 @@ static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
         }
     }
     auto fn = k_seg<RP, NO, FT, KIND>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)std::min<size_t>(c->max_lds, 160u * 1024u));
-        attr_set = true;
-    }
+    static std::atomic<uint64_t> lds_set{0};
+    lds_limit_once(c, &lds_set, (const void*)fn, (int)std::min<size_t>(c->max_lds, 160u * 1024u));
     a.nwaves = (uint32_t)c->num_cus * (uint32_t)(seg_threads<RP, NO, KIND>() / 64);
     if (c->launch_ev[0]) {
         hipExtLaunchKernelGGL(fn, dim3(c->num_cus), dim3(seg_threads<RP, NO, KIND>()),

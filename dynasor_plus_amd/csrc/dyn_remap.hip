@@ -317,13 +317,8 @@ int ensure_relabel(dyn_gpu_ctx* c) {
         if (c->nnz) {
             const uint32_t nblk = (uint32_t)std::min<uint64_t>(
                 std::max<uint64_t>((c->nnz + 65535) / 65536, 1), (uint64_t)c->num_cus * 4);
-            static bool attr_set = false;
-            if (!attr_set) {
-                (void)hipFuncSetAttribute((const void*)k_bincount,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)(kHistBins * sizeof(uint32_t)));
-                attr_set = true;
-            }
+            static std::atomic<uint64_t> lds_set{0};
+            lds_limit_once(c, &lds_set, (const void*)k_bincount, (int)(kHistBins * sizeof(uint32_t)));
             for (uint32_t lo = 0; lo < dim; lo += kHistBins) {
                 const uint32_t nb = std::min(kHistBins, dim - lo);
                 hipLaunchKernelGGL(k_bincount, dim3(nblk), dim3(kHistBlock), nb * sizeof(uint32_t),
