@@ -396,6 +396,61 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT
     }
 }
 
+// Quad broadcast from lane t, t known at compile time once the caller's loop
+// is unrolled (the switch folds to one DPP move).
+__device__ __forceinline__ uint32_t qb_at(uint32_t v, int t) {
+    switch (t) {
+        case 0: return qb<0>(v);
+        case 1: return qb<1>(v);
+        case 2: return qb<2>(v);
+        default: return qb<3>(v);
+    }
+}
+__device__ __forceinline__ float qbf_at(float v, int t) {
+    return __uint_as_float(qb_at(__float_as_uint(v), t));
+}
+
+// Nonzeros [S, S + J) of the quad's 16 in a hot-split piece whose operand-1
+// rows all sit in LDS (nonzero s is component s % 4 of lane s / 4).  The
+// operand-0 gathers of all J are issued first; then, per nonzero, its value
+// and operand-1 index are broadcast and its operand-1 row read from LDS just
+// before the accumulation.  Operand-1 rows thus hold no registers while the
+// gathers are in flight, and J = 6 memory gathers fit where the general
+// batch keeps 4 (plus 4 LDS rows).
+template <int RP, typename FT, int KIND, int S, int J, int CPL, typename Live>
+__device__ __forceinline__ void batch_hot(const SegArgs& a, const GatherBase<2, FT>& gb,
+                                          const float* sv4, const uint32_t (*si4)[4], int q,
+                                          float* acc, const float* arow, double* lsum,
+                                          uint32_t& prev0, uint32_t* raw0prev, Live live) {
+    constexpr int RD = Cols<RP, FT>::RD;
+    uint32_t raw0[J][RD];
+    uint32_t i0[J];
+    bool reuse[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        i0[j] = qb_at(si4[0][(S + j) % 4], (S + j) / 4);
+        const uint32_t before = (j == 0) ? prev0 : i0[j - 1];
+        reuse[j] = sizeof(FT) == 4 && i0[j] == before;
+        if (!reuse[j]) gather_raw<RP, FT>(gb.g[0], gb.l[0], a.hot[0], i0[j], raw0[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (live(j)) {
+            const uint32_t i1 = qb_at(si4[1][(S + j) % 4], (S + j) / 4);
+            const float v = qbf_at(sv4[(S + j) % 4], (S + j) / 4);
+            uint32_t r1[RD];
+            load_raw_as<RP, FT, 3>(gb.l[1] + (size_t)i1 * RP, r1);
+#pragma unroll
+            for (int d = 0; d < RD; ++d) raw0prev[d] = reuse[j] ? raw0prev[d] : raw0[j][d];
+            float y[2][CPL];
+            widen_raw<RP, FT>(raw0prev, y[0]);
+            widen_raw<RP, FT>(r1, y[1]);
+            accumulate<CPL, 2, KIND>(acc, v, y, arow, a.loss, lsum, q, a.loss_part != nullptr);
+            prev0 = i0[j];
+        }
+    }
+}
+
 // Contiguous split of P pieces over W waves: wave w owns
 // [range_begin(w), range_begin(w + 1)) with P = W q + rem, the first rem
 // waves taking q + 1 pieces.
@@ -539,7 +594,29 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                         [&](int j) { return k0 + (uint32_t)j < n; }, [&](int) {}, hot1);
                 }
             };
+            // FP32 hot pieces at Rp <= 32: nonzeros in parts of 6, 6 and 4
+            // (batch_hot; wider rows would spill its registers)
+            auto hot_part = [&](auto sc, auto jc) {
+                constexpr int S = decltype(sc)::value;
+                constexpr int JN = decltype(jc)::value;
+                const uint32_t k0 = (uint32_t)(16 * qw + S);
+                if (k0 < n) {
+                    if constexpr (NO == 2)
+                        batch_hot<RP, FT, KIND, S, JN, CPL>(
+                            a, gb, sv4, si4, q, acc, arow, &lsum, prev0, raw0prev,
+                            [&](int j) { return k0 + (uint32_t)j < n; });
+                }
+            };
             auto do_piece = [&](auto splitc) {
+                if constexpr (decltype(splitc)::value == 1 && sizeof(FT) == 4 && KIND == 0 &&
+                              RP <= 32) {
+                    if (hot1) {
+                        hot_part(std::integral_constant<int, 0>{}, std::integral_constant<int, 6>{});
+                        hot_part(std::integral_constant<int, 6>{}, std::integral_constant<int, 6>{});
+                        hot_part(std::integral_constant<int, 12>{}, std::integral_constant<int, 4>{});
+                        return;
+                    }
+                }
                 do_rounds(std::integral_constant<int, 0>{}, splitc);
                 if constexpr (NR == 1) do_rounds(std::integral_constant<int, 1>{}, splitc);
                 do_rounds(std::integral_constant<int, 2>{}, splitc);
