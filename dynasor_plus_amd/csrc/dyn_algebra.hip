@@ -31,6 +31,9 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int kMaxGramWG = 1024;
 constexpr int kSolveBlock = 256;
 constexpr int kMaxSolveWG = 1024;
+// relative rounding allowance of ||T||^2 - 2 <T, M> + ||M||^2 (FP32 spMTTKRP
+// inputs), measured against the largest of the three terms
+constexpr double kFitRoundRel = 1e-4;
 
 // ---------------------------------------------------------------------------
 //  FP64 reduction of ||vals||^2
@@ -611,14 +614,26 @@ __global__ __launch_bounds__(kBlock) void k_fit(FitArgs a) {
         for (int n = 0; n < a.N; ++n) h *= a.G[n][(size_t)r * a.RP + c];
         rs += a.lambda[r] * (a.lambda[c] * h);
     }
-    double mn = block_sum(rs, red);
+    const double mn = block_sum(rs, red);
     if (tid == 0) {
-        mn = fmax(mn, 0.0);
+        // ||T - M||^2 >= 0 holds exactly; a negative value within the rounding
+        // of its three terms is clamped to zero (the reference's clamp,
+        // src/dynasor_cpals.cpp:345-346).  A value further below zero, or a
+        // non-finite term, means the model has left the range where the three
+        // terms can be formed (a diverged update): the fit and residual are
+        // then reported as NaN rather than as a perfect fit.
         double r2 = a.tnorm2 - 2.0 * inner + mn;
-        if (r2 < 0.0) r2 = 0.0;
-        const double resid = sqrt(r2);
-        const double tn = sqrt(a.tnorm2);
-        a.out[0] = 1.0 - resid / fmax(tn, 1e-30);
+        const double scale = fmax(a.tnorm2, fmax(2.0 * fabs(inner), fabs(mn)));
+        const bool finite = isfinite(inner) && isfinite(mn) && isfinite(r2);
+        double fit, resid;
+        if (finite && r2 >= -kFitRoundRel * scale) {
+            resid = sqrt(fmax(r2, 0.0));
+            fit = 1.0 - resid / fmax(sqrt(a.tnorm2), 1e-30);
+        } else {
+            resid = __longlong_as_double(0x7FF8000000000000ll);
+            fit = resid;
+        }
+        a.out[0] = fit;
         a.out[1] = resid;
         a.out[2] = inner;
         a.out[3] = mn;
@@ -853,6 +868,17 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         }
         return mt;
     };
+    // a NaN fit marks a model whose fit terms are no longer finite or
+    // consistent (k_fit); it never counts as converged, and is reported once
+    bool warned = false;
+    auto warn_fit = [&](int iter, double fit) {
+        if (std::isfinite(fit) || warned) return;
+        warned = true;
+        std::fprintf(stderr,
+                     "[cpals] WARN: iteration %d: the model diverged (||T - M||^2 is not "
+                     "finite or is negative beyond rounding); fit reported as NaN\n",
+                     iter);
+    };
     int nev = 0;
     for (int iter = 0; iter < iters && rc == DYN_OK; ++iter) {
         auto t0 = clk::now();
@@ -920,6 +946,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         const double fit = c->fitbuf_host[0];
         const double resid = c->fitbuf_host[1];
         const double it = secs(t0, clk::now());
+        warn_fit(iter, fit);
         if (res->fit_history) res->fit_history[iter] = fit;
         if (res->iter_time_s) res->iter_time_s[iter] = it;
         res->final_fit = fit;
@@ -956,6 +983,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                 const double it =
                     hipEventElapsedTime(&ms, E[0], next) == hipSuccess ? ms * 1e-3 : 0.0;
                 const double fit = h[(size_t)iter * 4], resid = h[(size_t)iter * 4 + 1];
+                warn_fit(iter, fit);
                 if (res->fit_history) res->fit_history[iter] = fit;
                 if (res->iter_time_s) res->iter_time_s[iter] = it;
                 res->mttkrp_time_s += mttkrp_secs(E, nev);

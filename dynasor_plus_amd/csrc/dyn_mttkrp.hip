@@ -579,7 +579,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
             }
             // round t: nonzeros 16 g + 4 t + j, held by lane t of the quad
             // a hot-split piece reads operand 1 from LDS (hot) or memory (cold)
-            const bool hot1 = kSplitPath<NO, FT>() && a.pc_hot[p] != 0u;
+            const bool hot1 = kSplitPath<NO, FT>() && a.pc_hot && a.pc_hot[p] != 0u;
             auto do_rounds = [&](auto t0c, auto splitc) {
                 constexpr int T0 = decltype(t0c)::value;
                 constexpr int SP = decltype(splitc)::value;
@@ -1016,12 +1016,14 @@ __global__ __launch_bounds__(kBlock) void k_forward(FwdArgs a) {
     const int tid = threadIdx.x;
     const int g = tid >> 2;
     const int q = tid & 3;
-    float lam[CPL];
+    // lambda stays FP64: a diverging CP-ALS drives it past the FP32 range,
+    // and the model value is only formed after the FP32 factor products
+    double lam[CPL];
 #pragma unroll
     for (int b = 0; b < C::NB; ++b)
 #pragma unroll
         for (int t = 0; t < C::LW; ++t)
-            lam[b * C::LW + t] = a.lambda ? (float)a.lambda[C::col(b, q, t)] : 1.f;
+            lam[b * C::LW + t] = a.lambda ? a.lambda[C::col(b, q, t)] : 1.0;
     double dot = 0.0;
     const uint64_t per_block = (uint64_t)kQuads * 4;
     for (uint64_t k0 = (uint64_t)blockIdx.x * per_block + (uint64_t)g * 4; k0 < a.nnz;
@@ -1036,28 +1038,29 @@ __global__ __launch_bounds__(kBlock) void k_forward(FwdArgs a) {
         bcast_idx<1, N>(i_l, ib[1]);
         bcast_idx<2, N>(i_l, ib[2]);
         bcast_idx<3, N>(i_l, ib[3]);
-        float mj[4];
+        double mj[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float p[CPL];
+            load_row<RP, float>(a.A[0] + (size_t)ib[j][0] * RP, q, p);
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) p[c] = lam[c];
-#pragma unroll
-            for (int n = 0; n < N; ++n) {
+            for (int n = 1; n < N; ++n) {
                 float y[CPL];
                 load_row<RP, float>(a.A[n] + (size_t)ib[j][n] * RP, q, y);
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) p[c] *= y[c];
             }
-            float part = 0.f;
+            double part = 0.0;
 #pragma unroll
-            for (int c = 0; c < CPL; ++c) part += p[c];
-            mj[j] = quad_sum(part);
+            for (int c = 0; c < CPL; ++c) part = fma(lam[c], (double)p[c], part);
+            part += __shfl_xor(part, 1, 64);
+            part += __shfl_xor(part, 2, 64);
+            mj[j] = part;
         }
-        const float mine = (q == 0) ? mj[0] : (q == 1) ? mj[1] : (q == 2) ? mj[2] : mj[3];
+        const double mine = (q == 0) ? mj[0] : (q == 1) ? mj[1] : (q == 2) ? mj[2] : mj[3];
         if (kk < a.nnz) {
-            if (a.m) a.m[kk] = mine;
-            if (a.vals) dot += (double)v_l * (double)mine;
+            if (a.m) a.m[kk] = (float)mine;
+            if (a.vals) dot += (double)v_l * mine;
         }
     }
     if (a.dot_part) {

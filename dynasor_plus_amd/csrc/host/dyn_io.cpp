@@ -34,17 +34,28 @@ constexpr size_t kDnbHeader = 4 + 4 + 4 + 4 + 4 + 8 + 4 * DYN_MAX_MODES + 4 * 7;
 // ---------------------------------------------------------------------------
 Tensor::~Tensor() { free_elements(); }
 
-void Tensor::allocate_elements(uint64_t n, int nmodes) {
+bool Tensor::try_allocate_elements(uint64_t n, int nmodes) {
     free_elements();
     num_modes = nmodes;
     nnz = n;
-    vals = (value_t*)aligned_alloc64(n * sizeof(value_t));
-    for (int m = 0; m < nmodes; ++m) idx_buf[m] = (idx_t*)aligned_alloc64(n * sizeof(idx_t));
-    if (!vals) {
-        std::fprintf(stderr, "Tensor::allocate_elements: OOM for %llu nonzeros\n",
+    bool ok = (vals = (value_t*)aligned_alloc64(std::max<uint64_t>(n, 1) * sizeof(value_t)));
+    for (int m = 0; m < nmodes; ++m)
+        ok = (idx_buf[m] = (idx_t*)aligned_alloc64(std::max<uint64_t>(n, 1) * sizeof(idx_t))) && ok;
+    if (!ok) {
+        free_elements();
+        num_modes = 0;
+        nnz = 0;
+        std::fprintf(stderr, "Tensor::allocate_elements: out of host memory for %llu nonzeros\n",
                      (unsigned long long)n);
-        std::exit(1);
     }
+    return ok;
+}
+
+void Tensor::allocate_elements(uint64_t n, int nmodes) {
+    // the reference ends the process on a failed allocation
+    // (src/dynasor_io.cpp:149-212); the loaders and the C-ABI use
+    // try_allocate_elements and report the failure instead
+    if (!try_allocate_elements(n, nmodes)) std::exit(1);
 }
 
 void Tensor::free_elements() {
@@ -182,7 +193,7 @@ static bool load_tns_text(const std::string& path, Tensor& T) {
         std::fprintf(stderr, "ERROR: '%s' holds no nonzeros\n", path.c_str());
         return false;
     }
-    T.allocate_elements(vals.size(), nm);
+    if (!T.try_allocate_elements(vals.size(), nm)) return false;
     std::memcpy(T.vals, vals.data(), vals.size() * sizeof(float));
     for (int n = 0; n < nm; ++n) {
         std::memcpy(T.idx_buf[n], idx[n].data(), vals.size() * sizeof(idx_t));
@@ -240,7 +251,10 @@ static bool load_dnb(const std::string& path, Tensor& T) {
         std::fclose(fp);
         return false;
     }
-    T.allocate_elements(nnz, (int)nm);
+    if (!T.try_allocate_elements(nnz, (int)nm)) {
+        std::fclose(fp);
+        return false;
+    }
     for (uint32_t n = 0; n < nm; ++n) T.mode_size[n] = sizes[n];
     ok = std::fread(T.vals, 4, nnz, fp) == nnz;
     for (uint32_t n = 0; ok && n < nm; ++n) ok = std::fread(T.idx_buf[n], 4, nnz, fp) == nnz;

@@ -267,6 +267,16 @@ int main(int argc, char** argv) {
             cp.precision_source = "cli-precision";
         }
         dyn_print_compute_plan(cp, "[plan]");
+        if (!cp.fits) {
+            // the reference would stream from disk here (out-of-core Jacobi);
+            // the device engine needs the tensor and its copies resident
+            std::fprintf(stderr,
+                         "[plan] refusing: the device tensor and its copies need %.2f GiB, "
+                         "%.2f GiB of device memory are free\n",
+                         cp.device_total_bytes / 1073741824.0,
+                         cp.device_free_bytes / 1073741824.0);
+            return 6;
+        }
         opt.kernel = cp.kernel_enum == 1 ? CpKernel::Jacobi : CpKernel::GaussSeidel;
         opt.precision = cp.precision == 1 ? CpPrecision::BF16 : CpPrecision::FP32;
         std::printf("[cpals] kernel -> %s  (reason=%s; src=%s)\n",

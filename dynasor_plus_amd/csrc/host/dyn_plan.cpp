@@ -4,6 +4,7 @@
 // override, "auto"), sized against device memory instead of host RAM.
 #include "dynasor_common.h"
 #include "dynasor_gpu.h"
+#include "dynasor_jit.h"
 
 #include <algorithm>
 #include <cctype>
@@ -31,8 +32,22 @@ int device_rank(int R) {
 ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_threads,
                                    const std::string& layout_cli, const std::string& ooc_cli,
                                    const std::string& kernel_cli) {
-    (void)layout_cli;   // CPU layouts (PingPong / NCopy / InPlace / Morton) do not apply:
-    (void)ooc_cli;      // the device always builds one sorted copy per mode
+    uint64_t fr = 0, tot = 0;
+    const bool have_dev = dyn_gpu_device_memory(0, &fr, &tot) == DYN_OK;
+    // a context cap (DYN_GPU_MEM_CAP_MB) bounds what a run may allocate
+    if (const char* cap = std::getenv("DYN_GPU_MEM_CAP_MB")) {
+        const long long mb = std::atoll(cap);
+        if (mb > 0) fr = std::min<uint64_t>(fr, (uint64_t)mb << 20);
+    }
+    return dyn_build_compute_plan_with_memory(T, rank_padded, num_threads, layout_cli, ooc_cli,
+                                              kernel_cli, have_dev, fr, tot);
+}
+
+ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded, int num_threads,
+                                               const std::string& layout_cli,
+                                               const std::string& ooc_cli,
+                                               const std::string& kernel_cli, bool have_dev,
+                                               uint64_t fr, uint64_t tot) {
     ComputePlan p;
     const int N = T.num_modes;
     p.num_modes = N;
@@ -41,10 +56,12 @@ ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_thr
     const int Rd = device_rank(rank_padded);
     p.device_rank_padded = Rd;
 
-    size_t fb = 0, dfb = 0;
+    // ---- storage: device memory in place of host RAM --------------------
+    size_t fb = 0, dfb = 0, acc = 0;
     for (int n = 0; n < N; ++n) {
         fb += 2 * (size_t)T.mode_size[n] * rank_padded * sizeof(value_t);
-        dfb += (size_t)T.mode_size[n] * Rd * (4 + 2 + 4);    // Y, BF16 shadow, Yhat
+        dfb += (size_t)T.mode_size[n] * Rd * (4 + 4 + 2 + 4);   // Y, rank-order Y, BF16, Yhat
+        acc += (size_t)T.mode_size[n] * Rd * sizeof(float);
     }
     p.factor_bytes = fb;
     p.device_factor_bytes = dfb;
@@ -55,12 +72,35 @@ ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_thr
     p.device_tensor_bytes = tb;
     const size_t sort_scratch = (size_t)T.nnz * 12;
     p.device_total_bytes = tb + dfb + sort_scratch;
-    uint64_t fr = 0, tot = 0;
-    if (dyn_gpu_device_memory(0, &fr, &tot) == DYN_OK) {
+    if (!have_dev) fr = tot = 0;
+    if (have_dev) {
         p.device_free_bytes = fr;
         p.fits = p.device_total_bytes < fr;
     }
+    StoragePlan& sp = p.storage;
+    sp.choice = Layout::DeviceCopies;
+    sp.avail_bytes = fr;
+    sp.total_bytes = tot;
+    sp.budget_bytes = fr;
+    sp.ncopy_bytes = tb;
+    sp.overhead_bytes = dfb + sort_scratch;
+    sp.ncopy_need = p.device_total_bytes;
+    sp.reason = !have_dev ? "device memory unknown"
+                : p.fits  ? "per-mode sorted copies fit in device memory"
+                          : "per-mode sorted copies exceed free device memory";
+    // CPU layout / out-of-core requests have no device counterpart
+    const std::string lc = lower(layout_cli), oc = lower(ooc_cli);
+    sp.override_source = (!lc.empty() && lc != "auto") ? "cli-layout (ignored: device copies)"
+                         : (!oc.empty() && oc != "auto" && oc != "off")
+                             ? "cli-ooc (ignored: device copies)"
+                             : "auto";
 
+    // ---- kernel: the reference's decision tree ---------------------------
+    // (src/dynasor_preprocess.cpp:1253-1308) with the all-modes pass's
+    // rank-ordered accumulators in place of the per-thread private outputs
+    p.ofibs_bytes = acc;
+    const uint64_t cap8 = 8ull << 30;
+    const uint64_t budget = std::min<uint64_t>(fr / 4, cap8);
     std::string k = kernel_cli;
     const char* src = "auto";
     if (const char* env = std::getenv("DYN_CP_KERNEL"); env && env[0]) {
@@ -70,6 +110,22 @@ ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_thr
         src = "cli-kernel";
     }
     k = lower(k);
+    auto pick_auto = [&]() -> int {
+        if (T.nnz < 1000000ull) {
+            p.kernel_reason = "nnz<1M (linalg-dominated)";
+            return 0;
+        }
+        if ((N == 3 && Rd >= 64) || Rd >= 128) {
+            p.kernel_reason = "all-modes-fallback-shape";
+            return 0;
+        }
+        if (!have_dev || (uint64_t)acc >= budget) {
+            p.kernel_reason = "ofibs>budget";
+            return 0;
+        }
+        p.kernel_reason = "all-modes fits";
+        return 1;
+    };
     if (k == "jacobi" || k == "all-modes" || k == "all_modes" || k == "js") {
         p.kernel_enum = 1;
         p.kernel_reason = "user-forced-jacobi";
@@ -83,12 +139,12 @@ ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_thr
                          "warning: dyn_build_compute_plan: unknown kernel '%s'; falling back to "
                          "auto.\n",
                          kernel_cli.c_str());
-        // every per-mode copy is resident in HBM, so a Gauss-Seidel sweep costs
-        // the same tensor traffic as a Jacobi pass and converges faster
-        p.kernel_enum = 0;
-        p.kernel_reason = "per-mode copies resident in HBM";
+        p.kernel_enum = pick_auto();
     }
     p.kernel_source = src;
+    for (int n = 0; n < N && n < DYN_MAX_MODES; ++n) p.per_mode[n] = KernelImpl::FiberCSR;
+    p.pf_far = false;
+    p.use_jit = false;
 
     if (const char* env = std::getenv("DYN_CP_PRECISION"); env && env[0]) {
         const std::string e = lower(env);
@@ -98,9 +154,26 @@ ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_thr
     return p;
 }
 
+namespace {
+bool g_jit_forced = false;
+bool g_jit_on = false;
+}  // namespace
+
+bool dyn_jit_enabled() {
+    if (g_jit_forced) return g_jit_on;
+    const char* e = std::getenv("DYN_JIT");
+    return e && e[0] && e[0] != '0';
+}
+
+void dyn_jit_force_enable(bool on) {
+    g_jit_forced = true;
+    g_jit_on = on;
+}
+
 void dyn_print_compute_plan(const ComputePlan& p, const char* prefix) {
     const double GiB = 1024.0 * 1024.0 * 1024.0;
-    std::printf("%s layout : %s (one sorted copy per mode on the device)\n", prefix, p.layout);
+    std::printf("%s layout : %s (one sorted copy per mode on the device; reason=%s; src=%s)\n",
+                prefix, p.layout, p.storage.reason, p.storage.override_source);
     std::printf("%s device : tensor=%.2f GiB  factors=%.2f GiB  total=%.2f GiB  free=%.2f GiB%s\n",
                 prefix, p.device_tensor_bytes / GiB, p.device_factor_bytes / GiB,
                 p.device_total_bytes / GiB, p.device_free_bytes / GiB,
@@ -110,6 +183,10 @@ void dyn_print_compute_plan(const ComputePlan& p, const char* prefix) {
     std::printf("%s gather : %s  (src=%s)  Rp(host)=%d  Rp(device)=%d\n", prefix,
                 p.precision ? "bf16" : "fp32", p.precision_source, p.rank_padded,
                 p.device_rank_padded);
+    std::printf("%s per-mode : FiberCSR (segmented device spMTTKRP) x %d  pf_far=%d  jit=%d  "
+                "all-modes accumulators=%.1f MiB\n",
+                prefix, p.num_modes, (int)p.pf_far, (int)p.use_jit,
+                p.ofibs_bytes / (1024.0 * 1024.0));
 }
 
 }  // namespace dynasor

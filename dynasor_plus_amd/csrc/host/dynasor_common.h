@@ -25,6 +25,50 @@ namespace dynasor {
 using idx_t = uint32_t;    // include/dynasor_common.h:28
 using value_t = float;     // include/dynasor_common.h:30
 
+// Storage layouts (include/dynasor_common.h:113-165).  The device engine
+// always holds the tensor as one sorted copy per mode in device memory
+// (DeviceCopies); the CPU layouts keep their values so callers that read,
+// compare or assign a layout compile and behave unchanged.
+enum class Layout : uint8_t {
+    PingPong = 0,
+    NCopy = 1,
+    InPlace = 2,
+    OutOfCore = 3,
+    Morton = 4,
+    DeviceCopies = 5,   // per-mode sorted copies in HBM (dyn_gpu_remap)
+};
+
+// Storage decision (include/dynasor_common.h:202-224): the same fields, with
+// device memory in place of host RAM.  avail/total are the device's free and
+// total bytes, ncopy_bytes the device tensor footprint (the COO upload plus
+// one sorted copy per mode), overhead_bytes the factors, their BF16 shadow,
+// Yhat and the sort scratch, and ncopy_need their sum.
+struct StoragePlan {
+    Layout      choice = Layout::DeviceCopies;
+    size_t      avail_bytes = 0;
+    size_t      total_bytes = 0;
+    size_t      budget_bytes = 0;
+    size_t      pingpong_bytes = 0;
+    size_t      ncopy_bytes = 0;
+    size_t      inplace_bytes = 0;
+    size_t      overhead_bytes = 0;
+    size_t      pingpong_need = 0;
+    size_t      ncopy_need = 0;
+    size_t      inplace_need = 0;
+    size_t      ooc_need = 0;
+    const char* reason = "";
+    const char* override_source = "";
+};
+
+// Per-mode kernel implementation (include/dynasor_common.h:261-265).  The
+// device spMTTKRP walks a row pointer over a sorted copy, the counterpart of
+// FiberCSR, for every mode.
+enum class KernelImpl : uint8_t {
+    Element = 0,
+    Fiber = 1,
+    FiberCSR = 2,
+};
+
 // SoA COO tensor (include/dynasor_common.h:322-600, the fields cpals() and the
 // loaders use).  Owns vals / idx_buf; non-copyable like the reference.
 struct Tensor {
@@ -33,8 +77,10 @@ struct Tensor {
     idx_t    mode_size[DYN_MAX_MODES] = {0};
     value_t* vals = nullptr;
     idx_t*   idx_buf[DYN_MAX_MODES] = {nullptr};
+    Layout   layout = Layout::DeviceCopies;   // include/dynasor_common.h:390
 
-    void allocate_elements(uint64_t nnz, int nmodes);
+    void allocate_elements(uint64_t nnz, int nmodes);     // exits on host OOM (reference)
+    bool try_allocate_elements(uint64_t nnz, int nmodes); // false on host OOM
     void free_elements();
 
     Tensor() = default;
@@ -66,9 +112,14 @@ struct FactorMatrices {
 //  fields describe device memory instead of the CPU layouts.
 // ---------------------------------------------------------------------------
 struct ComputePlan {
+    StoragePlan storage;                  // device storage decision
     int         kernel_enum = 0;          // CpKernel value (0 = GS, 1 = Jacobi)
     const char* kernel_reason = "";
     const char* kernel_source = "auto";   // "cli-kernel" | "env-kernel" | "auto"
+    KernelImpl  per_mode[DYN_MAX_MODES] = {KernelImpl::Element};
+    bool        pf_far = false;           // no host prefetch tier on the device path
+    bool        use_jit = false;          // kernels are compiled ahead of time for gfx950
+    size_t      ofibs_bytes = 0;          // all-modes pass: rank-ordered Yhat accumulators
     int         precision = 0;            // 0 = FP32 gathers, 1 = BF16 gathers
     const char* precision_source = "auto";
     int         num_modes = 0;
@@ -87,6 +138,14 @@ struct ComputePlan {
 ComputePlan dyn_build_compute_plan(const Tensor& T, int rank_padded, int num_threads,
                                    const std::string& layout_cli, const std::string& ooc_cli,
                                    const std::string& kernel_cli);
+// dyn_build_compute_plan with the device memory given instead of queried
+// (device 0, capped by DYN_GPU_MEM_CAP_MB): `have_device` false plans as if
+// no device were visible (auto then keeps Gauss-Seidel and `fits` stays true).
+ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded, int num_threads,
+                                               const std::string& layout_cli,
+                                               const std::string& ooc_cli,
+                                               const std::string& kernel_cli, bool have_device,
+                                               uint64_t free_bytes, uint64_t total_bytes);
 void dyn_print_compute_plan(const ComputePlan& p, const char* prefix = "[plan]");
 
 // ---------------------------------------------------------------------------

@@ -333,7 +333,11 @@ int orc_cpals(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
 
 void orc_forward(int N, uint64_t nnz, const uint32_t* const* idx, int R, int ld,
                  const double* const* A, const double* lambda, double* m_out) {
-    for (uint64_t k = 0; k < nnz; ++k) {
+    // every m_out[k] is formed by one thread in the same order, so the
+    // result does not depend on the thread count
+#pragma omp parallel for schedule(static)
+    for (int64_t kk = 0; kk < (int64_t)nnz; ++kk) {
+        const uint64_t k = (uint64_t)kk;
         double m = 0.0;
         for (int r = 0; r < R; ++r) {
             double p = lambda ? lambda[r] : 1.0;

@@ -74,21 +74,6 @@ double orc_completion_loss(int N, uint64_t nnz, const uint32_t* const* idx,
                            const float* y, int R, int ld, const double* const* A,
                            int loss);
 
-/* CPU baseline (cpu_baseline.cpp): OpenMP per-mode CSR spMTTKRP on sorted
- * copies + FP32 CP-ALS algebra; times in seconds. */
-typedef struct orc_bench_result {
-    double preprocess_s;
-    double iter_s_median;
-    double mttkrp_sweep_s_median;
-    int    threads;
-    int    iters;
-    double final_fit;
-} orc_bench_result;
-
-int orc_cpu_cpals_bench(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
-                        const uint32_t* const* idx, int R, int iters, uint64_t seed,
-                        int threads, orc_bench_result* out);
-
 #ifdef __cplusplus
 }
 #endif

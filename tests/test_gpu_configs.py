@@ -29,6 +29,18 @@ def say(*a):
     sys.stdout.flush()
 
 
+def tensor_checksum(T):
+    """SHA-256 over dims, values and index columns (the recipe record's
+    checksum, tests/golden/make_golden.py)."""
+    import hashlib
+    h = hashlib.sha256()
+    h.update(np.array(T.dims, dtype=np.uint32).tobytes())
+    h.update(np.ascontiguousarray(T.vals, dtype=np.float32).tobytes())
+    for c in T.idx:
+        h.update(np.ascontiguousarray(c, dtype=np.uint32).tobytes())
+    return h.hexdigest()
+
+
 @pytest.fixture(scope="module")
 def c2():
     rec = json.loads((GOLDEN / "cpals_c2_gauss-seidel_r32.json").read_text())
@@ -69,6 +81,9 @@ def test_c2_mttkrp_column_invariant(c2):
 def test_c2_fits_track_reference(c2):
     say("c2 fits")
     T, rec = c2
+    # the tensor is regenerated from its recipe: it must be the one the
+    # reference ran on
+    assert tensor_checksum(T) == rec["checksum"]
     with GpuContext(0) as ctx:
         ctx.upload(T)
         ctx.set_factors(init_factors(T.dims, 32, rec["init_seed"], "normal"), 32)
@@ -110,12 +125,15 @@ def test_c3_shape_mixed_precision_equal_fit():
     assert abs(mixed.final_fit - plain.final_fit) < 5e-4, (mixed.final_fit, plain.final_fit)
 
 
-def test_c4_shape_jacobi_true_fit():
+@pytest.mark.parametrize("prec", [L.DYN_PREC_FP32, L.DYN_PREC_BF16])
+def test_c4_shape_jacobi_true_fit(prec):
     """Jacobi with the reference's update order (all spMTTKRPs from the
     iteration-start factors, Grams refreshed mode by mode) can diverge on this
     shape, so trajectories are only comparable at the first iteration; the fit
-    the device reports must then equal the true FP64 fit of its own model."""
-    say("c4 jacobi")
+    the device reports must then equal the true FP64 fit of its own model.
+    Both operand precisions: FP32 against the FP64 oracle, BF16 against the
+    FP64 oracle with BF16-rounded gathers (the configured C4 run)."""
+    say("c4 jacobi", "bf16" if prec else "fp32")
     orc = load_oracle()
     T, _, _ = host.generate((3000, 1000, 500, 80, 20), 150_000, seed=4, coords="zipf", alpha=1.0,
                             values="planted", planted_rank=20, noise_frac=0.05)
@@ -123,15 +141,18 @@ def test_c4_shape_jacobi_true_fit():
     # N(0,1) init keeps lambda moderate (U[0,1) init drives it to ~1e10 here,
     # where the fit's three-term difference cancels to noise)
     Y0 = init_factors(T.dims, R, 42, "normal")
-    fits64, _, _ = orc.cpals(T, Y0, 1, 0.0, jacobi=True, fp64=True)
+    fits64, _, _ = orc.cpals(T, Y0, 1, 0.0, jacobi=True, fp64=True, bf16_gather=bool(prec))
     with GpuContext(0) as ctx:
         ctx.upload(T)
         ctx.set_factors(Y0, R)
-        first = ctx.run_cpals(1, 0.0, kernel=L.DYN_CP_JACOBI)
+        first = ctx.run_cpals(1, 0.0, kernel=L.DYN_CP_JACOBI, precision=prec)
         ctx.set_factors(Y0, R)
-        res = ctx.run_cpals(4, 0.0, kernel=L.DYN_CP_JACOBI)
+        res = ctx.run_cpals(4, 0.0, kernel=L.DYN_CP_JACOBI, precision=prec)
         Y = [y.astype(np.float64) for y in ctx.get_factors()]
         lam = ctx.get_lambda()
+    say("fits", list(res.fit_history))
+    # a clamped residual would report a perfect fit for a diverged model
+    assert not any(f == 1.0 for f in res.fit_history)
     # FP32 spMTTKRP vs the FP64 oracle, amplified by the diverging update
     assert abs(first.fit_history[0] - fits64[0]) <= 1e-4 * max(1.0, abs(fits64[0]))
     v = T.vals.astype(np.float64)

@@ -8,9 +8,11 @@ do not need an oracle run over the whole tensor:
   * configuration 4 (5-mode 500k x 100k x 50k x 5k x 100, 400M Zipf(1.0)
     coordinates, planted rank 20 + 5 % noise, rank 32): row pointers equal the
     index bincounts in every mode, the column invariant over all five modes,
-    and one Jacobi and one Gauss-Seidel sweep run to finite fits and lambdas
-    (Jacobi diverges on this tensor within a few sweeps, a property of the
-    update order, so only its first sweep is checked);
+    one Jacobi and one Gauss-Seidel sweep run to finite fits and lambdas, and
+    the configured 30-iteration Jacobi run (BF16 gathers, and FP32) reports
+    no clamped fit of 1.0 and a final fit equal to the FP64 true fit of the
+    model it returns, or NaN where that fit is not finite (Jacobi diverges on
+    this tensor, a property of the reference's update order);
   * configuration 5 (1M x 100k x 2k, 200M uniform coordinates, Bernoulli
     labels from a planted rank-16 N(0, 0.5) model): 100 Adam steps at rank 32
     on the fused loss/adjoint, the final device loss against the FP64 oracle
@@ -97,8 +99,37 @@ def test_c4_full_remap_invariant_and_jacobi():
         lam_j = ctx.get_lambda()
         ctx.set_factors(Y0, R)
         gs = ctx.run_cpals(1, 0.0, kernel=L.DYN_CP_GAUSS_SEIDEL)
+        runs = {}
+        for prec in (L.DYN_PREC_BF16, L.DYN_PREC_FP32):
+            say("c4 full: 30 Jacobi iterations", "bf16" if prec else "fp32")
+            ctx.set_factors(Y0, R)
+            res = ctx.run_cpals(30, 0.0, kernel=L.DYN_CP_JACOBI, precision=prec)
+            runs[prec] = (res, [y.astype(np.float64) for y in ctx.get_factors()],
+                          ctx.get_lambda())
     assert np.isfinite(jac.final_fit) and np.isfinite(gs.final_fit)
     assert np.all(np.isfinite(lam_j)) and np.all(lam_j >= 0)
+    # the configured run: every reported fit is the true fit of the model or
+    # NaN (flagged divergence), never a clamped perfect fit; the last one is
+    # checked against the FP64 true fit of the returned model
+    orc = load_oracle()
+    v = T.vals.astype(np.float64)
+    t2 = float(v @ v)
+    for prec, (res, Y, lam) in runs.items():
+        say("fits", "bf16" if prec else "fp32", list(res.fit_history))
+        assert res.iters_done == 30
+        assert not any(f == 1.0 for f in res.fit_history)
+        say("c4 full: FP64 true fit of the returned model")
+        inner = float(v @ orc.forward(T, Y, lam))
+        H = np.ones((R, R))
+        for y in Y:
+            H *= y.T @ y
+        m2 = float(lam @ H @ lam)
+        r2 = t2 - 2 * inner + m2
+        if not (np.isfinite(r2) and r2 >= -1e-4 * max(t2, abs(2 * inner), abs(m2))):
+            assert np.isnan(res.final_fit), (res.final_fit, t2, inner, m2)
+            continue
+        fit = 1.0 - np.sqrt(max(r2, 0.0)) / np.sqrt(t2)
+        assert abs(res.final_fit - fit) <= 1e-5 * max(1.0, abs(fit)), (res.final_fit, fit)
 
 
 def test_c5_full_completion_adam_and_gradients():

@@ -217,6 +217,49 @@ def test_cpals_c1_tracks_reference(gpu_ctx):
         assert frob_rel(Y[n], z[f"final{n}"]) < 1e-3
 
 
+def test_cpals_3d_7_rank2_tracks_reference(gpu_ctx):
+    """The reference's own 9-nonzero tensor at rank 2 (padded to 16 rows of
+    zeros on the device): fits, final factors and lambda of its 10
+    Gauss-Seidel iterations."""
+    say("cpals 3d_7 rank 2")
+    z = golden("cpals_3d_7_gauss-seidel_r2.npz")
+    T = tensor_from_npz(z)
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors([z[f"init{n}"] for n in range(3)], 2)
+    res = gpu_ctx.run_cpals(int(z["iters"]), float(z["tol"]))
+    np.testing.assert_allclose(res.fit_history, z["fits"], rtol=0, atol=1.5e-6)
+    Y = gpu_ctx.get_factors()
+    for n in range(3):
+        assert frob_rel(Y[n], z[f"final{n}"]) < 1e-4
+    np.testing.assert_allclose(res.lambda_, z["lam"], rtol=1e-4)
+
+
+def test_cpals_singular_gram_zero_pivots(gpu_ctx, oracle):
+    """Two identical factor columns with small-integer entries and no ridge:
+    every Gram, Hadamard product and Cholesky step of the first solve is exact,
+    the second pivot is exactly zero, and the reference's rule (zero pivot ->
+    zero component, src/dynasor_linalg.cpp:186-189, 235, 243) zeroes that
+    column for good.  The device's explicit inverse must agree with the
+    oracle's per-row substitutions, and column 1 must stay exactly zero."""
+    say("cpals singular gram")
+    z = golden("cpals_3d_7_gauss-seidel_r2.npz")
+    T = tensor_from_npz(z)
+    base = [np.array([3, 1, 0, 2], np.float32), np.array([1, 0, 0, 0], np.float32),
+            np.array([2, 0, 0], np.float32)]
+    Y0 = [np.stack([b, b], axis=1) for b in base]      # G_1 o G_2 = [[4, 4], [4, 4]]
+    fits, Yo, lam = oracle.cpals(T, Y0, 4, 0.0, ridge=0.0, fp64=True)
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y0, 2)
+    res = gpu_ctx.run_cpals(4, 0.0, ridge=0.0)
+    Y = gpu_ctx.get_factors()
+    np.testing.assert_allclose(res.fit_history, fits, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(res.lambda_, lam, rtol=1e-5, atol=0)
+    assert res.lambda_[1] == 0.0 and lam[1] == 0.0
+    for n in range(3):
+        assert np.all(Y[n][:, 1] == 0.0)
+        np.testing.assert_allclose(Y[n], Yo[n], rtol=0, atol=1e-6)
+
+
 @pytest.mark.parametrize("jacobi", [False, True])
 def test_cpals_vs_fp64_oracle(gpu_ctx, oracle, jacobi):
     say("cpals fp64", jacobi)
