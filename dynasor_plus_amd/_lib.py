@@ -82,6 +82,7 @@ SIGNATURES = {
     "dyn_gpu_get_factors": (C.c_int, [_vp, C.c_int, _f32p, C.c_int]),
     "dyn_gpu_get_lambda": (C.c_int, [_vp, _f64p]),
     "dyn_gpu_mttkrp": (C.c_int, [_vp, C.c_int, C.c_int, _f32p, C.c_int]),
+    "dyn_gpu_mttkrp_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int]),
     "dyn_gpu_mttkrp_all": (C.c_int, [_vp, C.c_int, C.POINTER(_f32p), C.c_int]),
     "dyn_gpu_gram": (C.c_int, [_vp, C.c_int, _f64p]),
     "dyn_gpu_cpals": (C.c_int, [_vp, C.POINTER(CpOpts), C.POINTER(CpResultC)]),

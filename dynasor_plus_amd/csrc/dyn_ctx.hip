@@ -5,8 +5,45 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace dyn {
+
+// ---------------------------------------------------------------------------
+//  Fixed device scratch of dyn_gpu_mttkrp_noalloc: the heavy-row piece
+//  partials live in this file-scope array instead of context scratch, so
+//  that entry point allocates nothing.  Every GPU the library runs on holds
+//  its own copy until the process ends.  Host code never passes the array's
+//  name to a kernel: the name is a host-side stand-in, not a device address.
+//  Each device's address comes from hipGetSymbolAddress (once per device, at
+//  context creation, with that device current) and is what kernels receive.
+// ---------------------------------------------------------------------------
+namespace fixed {
+constexpr size_t kBytes = DYN_GPU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) float scratch[kBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_mu;
+static void* g_addr[kMaxDevices] = {nullptr};
+
+// Device address of `device`'s array; `device` must be current.
+static int address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return DYN_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return DYN_ERR_HIP;
+        }
+        g_addr[device] = p;
+    }
+    *out = g_addr[device];
+    return DYN_OK;
+}
+}  // namespace fixed
 
 int set_err(dyn_gpu_ctx* c, int code, const char* fmt, ...) {
     if (!c) return code;
@@ -239,6 +276,9 @@ extern "C" int dyn_gpu_create(int device, dyn_gpu_ctx** out) {
         long long mb = std::atoll(cap);
         if (mb > 0) c->mem_cap = (uint64_t)mb << 20;
     }
+    // the fixed partials array of this device (dyn_gpu_mttkrp_noalloc); on
+    // failure that entry point reports DYN_ERR_STATE, the rest is unaffected
+    if (fixed::address(device, &c->fixed_scratch) != DYN_OK) c->fixed_scratch = nullptr;
     if (hipHostMalloc((void**)&c->fitbuf_host, 16 * sizeof(double), hipHostMallocDefault) !=
         hipSuccess) {
         (void)hipStreamDestroy(c->aux_stream);
@@ -578,6 +618,36 @@ extern "C" int dyn_gpu_mttkrp(dyn_gpu_ctx* c, int mode, int prec, float* out, in
     DYN_CHECK(mttkrp_mode(c, mode, prec, Yin, nullptr, c->Yhat[mode], c->Rp));
     if (out) DYN_CHECK(copy_out_rows(c, c->Yhat[mode], c->dims[mode], c->Rp, c->R, out, ld));
     return DYN_OK;
+}
+
+extern "C" int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* c, int mode, int prec, float* Yhat_dev,
+                                      int ld) {
+    if (!c || !Yhat_dev) return DYN_ERR_INVALID;
+    if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
+    if (prec != DYN_PREC_FP32 && prec != DYN_PREC_BF16)
+        return set_err(c, DYN_ERR_INVALID, "bad precision %d", prec);
+    if (ld != c->Rp)
+        return set_err(c, DYN_ERR_INVALID, "ld must equal the padded rank %d (got %d)", c->Rp, ld);
+    if (!c->fixed_scratch)
+        return set_err(c, DYN_ERR_STATE, "no fixed scratch address on device %d", c->device);
+    DYN_HIP(c, hipSetDevice(c->device));
+    // everything this call needs must exist already: building or re-sorting
+    // the copy, or the frequency relabel, would allocate
+    if (!c->copy[mode].built || !c->relabeled)
+        return set_err(c, DYN_ERR_STATE, "mode %d copy not built (call dyn_gpu_remap first)", mode);
+    if (!split_ready(c, mode, c->Rp))
+        return set_err(c, DYN_ERR_STATE,
+                       "mode %d copy is not in launch order (run dyn_gpu_mttkrp on it once)", mode);
+    const size_t need = seg_part_bytes(c, mode, c->Rp);
+    if (need > fixed::kBytes)
+        return set_err(c, DYN_ERR_UNSUPPORTED,
+                       "mode %d needs %zu bytes of partials; the fixed array holds %zu", mode, need,
+                       fixed::kBytes);
+    const void* Yin[kMaxModes];
+    DYN_CHECK(gathered_operands(c, prec, Yin));
+    return mttkrp_mode(c, mode, prec, Yin, nullptr, Yhat_dev, c->Rp, (float*)c->fixed_scratch);
 }
 
 extern "C" int dyn_gpu_mttkrp_all(dyn_gpu_ctx* c, int prec, float* const* out, int ld) {

@@ -132,6 +132,10 @@ struct dyn_gpu_ctx {
     int          num_cus = 256;
     size_t       max_lds = 65536;
 
+    // device address of this device's fixed partials array (no-allocation
+    // entry point), from hipGetSymbolAddress at context creation
+    void*        fixed_scratch = nullptr;
+
     // scratch (grown on demand)
     void*        scratch = nullptr;
     size_t       scratch_bytes = 0;
@@ -203,8 +207,15 @@ int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H);
 // (I_mode x Rp floats, device).  `Yin[w]` are the gathered factors (FP32 or
 // BF16 according to `prec`; entry `mode` ignored), `vals_override` replaces
 // the copy's values with vals_override[perm[k]] when non-null.
+// `part_buf` (nullable): device memory for the heavy-row piece partials, at
+// least seg_part_bytes(c, mode, Rp) bytes; the call then allocates nothing and
+// requires split_ready(c, mode, Rp).
 int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
-                const float* vals_override, float* out, int Rp);
+                const float* vals_override, float* out, int Rp, float* part_buf = nullptr);
+size_t seg_part_bytes(const dyn_gpu_ctx* c, int mode, int Rp);
+// Whether copy `mode` is already in the order a launch at padded rank Rp
+// uses (hot split on 3-mode tensors, plain order otherwise).
+bool split_ready(const dyn_gpu_ctx* c, int mode, int Rp);
 // Adjoint of the forward model for mode `mode`: the spMTTKRP of the
 // per-nonzero vector g (input order) with caller factors A (input order,
 // Rp-strided device rows), into out (I_mode x Rp).

@@ -872,8 +872,28 @@ static int launch_heavy(dyn_gpu_ctx* c, int Rp, const ModeCopy& cp, const float*
     return check_launch(c, "heavy_rows");
 }
 
+size_t seg_part_bytes(const dyn_gpu_ctx* c, int mode, int Rp) {
+    const size_t pieces = std::max<uint32_t>(c->copy[mode].npieces, 1);
+    return ((pieces * Rp * sizeof(float)) + 255) & ~size_t(255);
+}
+
+bool split_ready(const dyn_gpu_ctx* c, int mode, int Rp) {
+    const ModeCopy& cp = c->copy[mode];
+    if (!cp.built) return false;
+    if (c->N < 3) return true;
+    if (c->N > 3) return cp.hot_w < 0;
+    int others[kMaxModes], j = 0;
+    for (int w = 0; w < c->N; ++w)
+        if (w != mode) others[j++] = w;
+    return cp.hot_w == others[1] &&
+           cp.hot_h == hot_split_rows(const_cast<dyn_gpu_ctx*>(c), Rp, c->dims[others[1]]);
+}
+
+// Segment-kernel arguments of copy `mode`.  The piece partials go to
+// `part_buf` when given (at least seg_part_bytes(c, mode, Rp) bytes, no loss
+// slots), else to the context's second scratch buffer.
 static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin, SegArgs* a,
-                       int* NO, uint32_t* dims_o, bool loss_slots) {
+                       int* NO, uint32_t* dims_o, bool loss_slots, float* part_buf = nullptr) {
     const ModeCopy& cp = c->copy[mode];
     if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
     *a = SegArgs();
@@ -900,9 +920,14 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     a->empty_rows = cp.empty_rows;
     a->nempty = cp.nempty;
     a->stream_bytes = (uint32_t)std::min<uint64_t>(c->nnz * 4ull, 0xFFFFFFF0ull);
+    if (part_buf) {
+        a->part = part_buf;
+        a->loss_part = nullptr;
+        a->loss_dev = nullptr;
+        return DYN_OK;
+    }
     void* sp = nullptr;
-    const size_t pieces = std::max<uint32_t>(cp.npieces, 1);
-    const size_t pbytes = ((pieces * Rp * sizeof(float)) + 255) & ~size_t(255);
+    const size_t pbytes = seg_part_bytes(c, mode, Rp);
     const size_t nloss = (size_t)cp.npieces + cp.nseg;
     const size_t lbytes = loss_slots ? ((std::max<size_t>(nloss, 1) * sizeof(double) + 255) & ~size_t(255)) : 0;
     DYN_CHECK(scratch2(c, pbytes + lbytes + 256, &sp));
@@ -913,13 +938,20 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
 }
 
 int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
-                const float* vals_override, float* out, int Rp) {
+                const float* vals_override, float* out, int Rp, float* part_buf) {
     SegArgs a;
     int NO = 0;
     uint32_t dims_o[kMaxModes];
     if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
-    DYN_CHECK(prepare_split(c, mode, Rp));
-    DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false));
+    if (part_buf) {
+        // caller-provided partials: the copy must already be in launch order
+        // (re-sorting it would allocate)
+        if (!split_ready(c, mode, Rp))
+            return set_err(c, DYN_ERR_STATE, "mode %d copy is not prepared for this launch", mode);
+    } else {
+        DYN_CHECK(prepare_split(c, mode, Rp));
+    }
+    DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false, part_buf));
     if (vals_override) {
         a.vals = vals_override;
         a.vperm = c->copy[mode].perm;
@@ -931,7 +963,7 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
     // timing events (the CP-ALS loop's pair, or a pool pair) are recorded by
     // the launch itself
     TimedPair tp;
-    const bool pooled = !c->seg_pair && timing_acquire(c, &tp);
+    const bool pooled = !c->seg_pair && !part_buf && timing_acquire(c, &tp);
     if (c->seg_pair) {
         c->launch_ev[0] = c->seg_pair[0];
         c->launch_ev[1] = c->seg_pair[1];

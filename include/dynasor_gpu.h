@@ -145,6 +145,22 @@ int dyn_gpu_get_lambda(dyn_gpu_ctx* ctx, double* lambda);
  * NULL to leave the result on the device only. */
 int dyn_gpu_mttkrp(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_out,
                    int ld);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dyn_gpu_mttkrp into the caller's device buffer Yhat_dev
+ * (I_mode x ld floats, ld equal to the context's padded rank), with the
+ * heavy-row partial sums in a fixed per-GPU device array of
+ * DYN_GPU_NOALLOC_SCRATCH_BYTES instead of context scratch.  Allocates
+ * nothing: the mode's copy must already be built and in launch order (an
+ * earlier dyn_gpu_remap and dyn_gpu_mttkrp of that mode at this rank), else
+ * DYN_ERR_STATE; a copy whose partials exceed the array returns
+ * DYN_ERR_UNSUPPORTED.  Nothing in the array needs clearing between calls
+ * (every partial is written before it is read), but all calls on one GPU
+ * share it: calls that overlap on one GPU (several contexts or streams)
+ * race.  Runs asynchronously on the context stream. */
+#define DYN_GPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_dev,
+                           int ld);
 /* Every Yhat_n from one snapshot of the factors in one tensor pass
  * (Jacobi all-modes).  Yhat_out may be NULL, or an array of N host
  * pointers (entries may be NULL). */
