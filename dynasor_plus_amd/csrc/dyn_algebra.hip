@@ -14,8 +14,10 @@
 //               refreshes the BF16 shadow and the Gram partials, and for the
 //               last mode the <Y, Yhat> column partials of the fit.
 //   Fit         ||T||^2 - 2 <T, M> + ||M||^2 in FP64 with the Yhat identity
-//               (Gauss-Seidel) or a forward-model pass (Jacobi)
-//               (src/dynasor_cpals.cpp:79-113, 332-373).
+//               (src/dynasor_cpals.cpp:79-113, 332-373): from the last mode's
+//               Yhat under Gauss-Seidel; under Jacobi from the next sweep's
+//               Yhat of the smallest mode, which reads exactly the model being
+//               measured (the reference's Jacobi print uses a stale Yhat).
 #include "dyn_internal.hpp"
 
 #include <algorithm>
@@ -824,7 +826,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     const int iters = std::max(o->max_iters, 0);
     const bool deferred = !(o->tol > 0.0) && iters > 0;
     const int per = 2 * N + 1;   // iteration start, then a begin/end pair per spMTTKRP
-    std::vector<hipEvent_t> evs((size_t)(deferred ? iters : 1) * per + 1, nullptr);
+    std::vector<hipEvent_t> evs((size_t)std::max(iters, 1) * per + 1, nullptr);
     double* hist_dev = nullptr;
     // the Hadamard / Cholesky / inverse of mode n depends only on the Grams,
     // so it runs on the auxiliary stream while mode n's spMTTKRP runs
@@ -879,122 +881,167 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                      "finite or is negative beyond rounding); fit reported as NaN\n",
                      iter);
     };
-    int nev = 0;
-    for (int iter = 0; iter < iters && rc == DYN_OK; ++iter) {
-        auto t0 = clk::now();
-        hipEvent_t* E = &evs[(size_t)(deferred ? iter : 0) * per];
-        (void)hipEventRecord(E[0], c->stream);
-        const void* Yin[kMaxModes];
-        for (int m = 0; m < N; ++m) Yin[m] = bf ? (const void*)c->Ygb[m] : (const void*)c->Yg[m];
-        nev = 0;
-        if (jacobi) {
-            (void)hipEventRecord(E[1], c->stream);
-            for (int m = 0; m < N && rc == DYN_OK; ++m)
-                rc = mttkrp_mode(c, m, o->precision, Yin, nullptr, c->Yhat[m], c->Rp);
-            (void)hipEventRecord(E[2], c->stream);
-            nev = 1;
-        }
+    // Jacobi: the fit of iteration t is formed from the spMTTKRPs of iteration
+    // t + 1, which read exactly the model of iteration t: for any mode s,
+    // <T, M_t> = sum_r lambda_r <Y_s[:, r], Yhat_s[:, r]> with Yhat_s taken from
+    // the factors of M_t (the Gauss-Seidel identity with a consistent Yhat).
+    // So no forward pass is needed per iteration: the fit of iteration t is
+    // evaluated after the next sweep (before any factor of M_t is replaced),
+    // and the last iteration's fit costs one extra spMTTKRP of mode s.
+    int s_mode = 0;
+    for (int m = 1; m < N; ++m)
+        if (c->dims[m] < c->dims[s_mode]) s_mode = m;
+    const void* Yin[kMaxModes];
+    const void* Yin32[kMaxModes];
+    for (int m = 0; m < N; ++m) {
+        Yin[m] = bf ? (const void*)c->Ygb[m] : (const void*)c->Yg[m];
+        Yin32[m] = c->Yg[m];
+    }
+    // With BF16 gathers the mode whose Yhat carries the fit (Gauss-Seidel: the
+    // last mode; Jacobi: s_mode) still gathers FP32 rows, so the reported fit
+    // is the fit of the FP32 model that the run returns.
+    const int fit_mode = jacobi ? s_mode : N - 1;
+    auto mttkrp_n = [&](int n) {
+        const bool f32 = bf && n == fit_mode;
+        return mttkrp_mode(c, n, f32 ? DYN_PREC_FP32 : o->precision, f32 ? Yin32 : Yin, nullptr,
+                           c->Yhat[n], c->Rp);
+    };
+    // fit of the current device model from Yhat[s_mode] (Jacobi) into `out`
+    auto jacobi_fit = [&](double* out) -> int {
         double* ipart = nullptr;
         uint32_t inwg = 0;
-        for (int n = 0; n < N && rc == DYN_OK; ++n) {
+        DYN_CHECK(scale_gram(c, s_mode, nullptr, true, &ipart, &inwg));
+        return fit_eval(c, ipart, inwg, inner_dev, tn2, out);
+    };
+    // per-mode updates of one iteration (Gauss-Seidel runs each spMTTKRP
+    // here, Jacobi before); Gauss-Seidel's last mode also forms the fit
+    auto update_modes = [&](hipEvent_t* E, int& nev, double* fit_out) -> int {
+        double* ipart = nullptr;
+        uint32_t inwg = 0;
+        for (int n = 0; n < N; ++n) {
             // Grams current -> V_n^{-1} on the auxiliary stream
             if (hipEventRecord(ev_g, c->stream) != hipSuccess ||
-                hipStreamWaitEvent(c->aux_stream, ev_g, 0) != hipSuccess) {
-                rc = set_err(c, DYN_ERR_HIP, "event record/wait failed");
-                break;
-            }
-            if ((rc = chol(c, n, o->ridge, c->aux_stream)) != DYN_OK) break;
-            if (hipEventRecord(ev_l, c->aux_stream) != hipSuccess) {
-                rc = set_err(c, DYN_ERR_HIP, "event record failed");
-                break;
-            }
+                hipStreamWaitEvent(c->aux_stream, ev_g, 0) != hipSuccess)
+                return set_err(c, DYN_ERR_HIP, "event record/wait failed");
+            DYN_CHECK(chol(c, n, o->ridge, c->aux_stream));
+            if (hipEventRecord(ev_l, c->aux_stream) != hipSuccess)
+                return set_err(c, DYN_ERR_HIP, "event record failed");
             if (!jacobi) {
                 c->seg_pair = &E[1 + 2 * n];
-                rc = mttkrp_mode(c, n, o->precision, Yin, nullptr, c->Yhat[n], c->Rp);
+                const int mrc = mttkrp_n(n);
                 c->seg_pair = nullptr;
                 nev = n + 1;
-                if (rc != DYN_OK) break;
+                DYN_CHECK(mrc);
             }
-            if (hipStreamWaitEvent(c->stream, ev_l, 0) != hipSuccess) {
-                rc = set_err(c, DYN_ERR_HIP, "stream wait failed");
-                break;
-            }
+            if (hipStreamWaitEvent(c->stream, ev_l, 0) != hipSuccess)
+                return set_err(c, DYN_ERR_HIP, "stream wait failed");
             SolveParts sq;
-            if ((rc = solve(c, n, &sq)) != DYN_OK) break;
+            DYN_CHECK(solve(c, n, &sq));
             const bool last = (n == N - 1) && !jacobi;
-            rc = scale_gram(c, n, &sq, last, last ? &ipart : nullptr, last ? &inwg : nullptr);
+            DYN_CHECK(scale_gram(c, n, &sq, last, last ? &ipart : nullptr, last ? &inwg : nullptr));
         }
-        if (rc != DYN_OK) break;
-        if (jacobi) {
-            // true <T, M> from the forward model at the stored coordinates
-            float* Ys[kMaxModes];
-            for (int m = 0; m < N; ++m) Ys[m] = c->Y[m];
-            rc = forward_eval(c, Ys, c->Rp, c->lambda, nullptr, c->vals, inner_dev);
-            if (rc != DYN_OK) break;
-        }
-        double* fit_out = deferred ? hist_dev + (size_t)iter * 4 : c->fitbuf;
-        if ((rc = fit_eval(c, ipart, inwg, inner_dev, tn2, fit_out)) != DYN_OK) break;
-        if (deferred) continue;
+        if (!jacobi) return fit_eval(c, ipart, inwg, inner_dev, tn2, fit_out);
+        return DYN_OK;
+    };
+    auto read_fit = [&](int iter) -> int {
         hipError_t e = hipMemcpyAsync(c->fitbuf_host, c->fitbuf, 4 * sizeof(double),
                                       hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) {
-            rc = set_err(c, DYN_ERR_HIP, "cpals iteration %d failed: %s", iter,
-                         hipGetErrorString(e));
-            break;
-        }
-        const double fit = c->fitbuf_host[0];
-        const double resid = c->fitbuf_host[1];
-        const double it = secs(t0, clk::now());
+        if (e != hipSuccess)
+            return set_err(c, DYN_ERR_HIP, "cpals iteration %d failed: %s", iter,
+                           hipGetErrorString(e));
+        return DYN_OK;
+    };
+    // one iteration's results, in order; returns true when it converged
+    auto record = [&](int iter, double fit, double resid, double it, const hipEvent_t* E,
+                      int nev) -> bool {
         warn_fit(iter, fit);
         if (res->fit_history) res->fit_history[iter] = fit;
         if (res->iter_time_s) res->iter_time_s[iter] = it;
+        res->mttkrp_time_s += mttkrp_secs(E, nev);
         res->final_fit = fit;
         res->final_residual = resid;
         res->iters_done = iter + 1;
-        res->mttkrp_time_s += mttkrp_secs(E, nev);
-        if (o->verbose) {
+        if (o->verbose)
             std::printf("[cpals] iter %3d  fit = %.6f  dfit = %.2e  resid = %.3e  iter = %.3f s\n",
                         iter, fit, iter == 0 ? 0.0 : std::fabs(fit - fit_prev), resid, it);
-        }
-        if (iter > 0 && std::fabs(fit - fit_prev) < o->tol) {
-            if (o->verbose)
-                std::printf("[cpals] converged after %d iterations (|dfit|=%.2e < %.1e)\n",
-                            iter + 1, std::fabs(fit - fit_prev), o->tol);
-            break;
-        }
+        const bool conv = !deferred && iter > 0 && std::fabs(fit - fit_prev) < o->tol;
+        if (conv && o->verbose)
+            std::printf("[cpals] converged after %d iterations (|dfit|=%.2e < %.1e)\n", iter + 1,
+                        std::fabs(fit - fit_prev), o->tol);
         fit_prev = fit;
+        return conv;
+    };
+    auto ev_ms = [](hipEvent_t a, hipEvent_t b) {
+        float ms = 0.f;
+        return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms * 1e-3 : 0.0;
+    };
+    hipEvent_t E_end = evs.back();
+    int nev = jacobi ? 1 : 0;
+    int done = 0;           // iterations whose factor updates ran
+    bool stopped = false;   // converged (Jacobi: before the updates of `done`)
+    for (int iter = 0; iter < iters && rc == DYN_OK && !stopped; ++iter) {
+        auto t0 = clk::now();
+        hipEvent_t* E = &evs[(size_t)iter * per];
+        (void)hipEventRecord(E[0], c->stream);
+        if (jacobi) {
+            (void)hipEventRecord(E[1], c->stream);
+            for (int m = 0; m < N && rc == DYN_OK; ++m) rc = mttkrp_n(m);
+            (void)hipEventRecord(E[2], c->stream);
+            if (rc != DYN_OK) break;
+            if (iter > 0) {
+                // the fit of the previous iteration's model
+                double* out = deferred ? hist_dev + (size_t)(iter - 1) * 4 : c->fitbuf;
+                if ((rc = jacobi_fit(out)) != DYN_OK) break;
+                if (!deferred) {
+                    if ((rc = read_fit(iter)) != DYN_OK) break;
+                    const hipEvent_t* Ep = &evs[(size_t)(iter - 1) * per];
+                    if (record(iter - 1, c->fitbuf_host[0], c->fitbuf_host[1], ev_ms(Ep[0], E[0]),
+                               Ep, nev)) {
+                        stopped = true;
+                        break;
+                    }
+                }
+            }
+            if ((rc = update_modes(E, nev, nullptr)) != DYN_OK) break;
+            done = iter + 1;
+            continue;
+        }
+        double* fit_out = deferred ? hist_dev + (size_t)iter * 4 : c->fitbuf;
+        if ((rc = update_modes(E, nev, fit_out)) != DYN_OK) break;
+        done = iter + 1;
+        if (deferred) continue;
+        if ((rc = read_fit(iter)) != DYN_OK) break;
+        stopped = record(iter, c->fitbuf_host[0], c->fitbuf_host[1], secs(t0, clk::now()), E, nev);
     }
-    if (deferred && rc == DYN_OK) {
-        hipEvent_t E_end = evs.back();
-        std::vector<double> h((size_t)iters * 4);
-        hipError_t e = hipEventRecord(E_end, c->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(h.data(), hist_dev, h.size() * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream);
+    if (rc == DYN_OK && done > 0) (void)hipEventRecord(E_end, c->stream);
+    // Jacobi: the last updated model's fit, from one more spMTTKRP of s_mode
+    if (rc == DYN_OK && jacobi && !stopped && done > 0) {
+        double* out = deferred ? hist_dev + (size_t)(done - 1) * 4 : c->fitbuf;
+        rc = mttkrp_n(s_mode);
+        if (rc == DYN_OK) rc = jacobi_fit(out);
+        if (rc == DYN_OK && !deferred) {
+            rc = read_fit(done - 1);
+            if (rc == DYN_OK) {
+                const hipEvent_t* Ep = &evs[(size_t)(done - 1) * per];
+                (void)record(done - 1, c->fitbuf_host[0], c->fitbuf_host[1], ev_ms(Ep[0], E_end),
+                             Ep, nev);
+            }
+        }
+    }
+    if (deferred && rc == DYN_OK && done > 0) {
+        std::vector<double> h((size_t)done * 4);
+        hipError_t e = hipMemcpyAsync(h.data(), hist_dev, h.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {
             rc = set_err(c, DYN_ERR_HIP, "cpals failed: %s", hipGetErrorString(e));
         } else {
-            for (int iter = 0; iter < iters; ++iter) {
+            for (int iter = 0; iter < done; ++iter) {
                 const hipEvent_t* E = &evs[(size_t)iter * per];
-                const hipEvent_t next = (iter + 1 < iters) ? evs[(size_t)(iter + 1) * per] : E_end;
-                float ms = 0.f;
-                const double it =
-                    hipEventElapsedTime(&ms, E[0], next) == hipSuccess ? ms * 1e-3 : 0.0;
-                const double fit = h[(size_t)iter * 4], resid = h[(size_t)iter * 4 + 1];
-                warn_fit(iter, fit);
-                if (res->fit_history) res->fit_history[iter] = fit;
-                if (res->iter_time_s) res->iter_time_s[iter] = it;
-                res->mttkrp_time_s += mttkrp_secs(E, nev);
-                res->final_fit = fit;
-                res->final_residual = resid;
-                res->iters_done = iter + 1;
-                if (o->verbose)
-                    std::printf("[cpals] iter %3d  fit = %.6f  dfit = %.2e  resid = %.3e  "
-                                "iter = %.3f s\n",
-                                iter, fit, iter == 0 ? 0.0 : std::fabs(fit - fit_prev), resid, it);
-                fit_prev = fit;
+                const hipEvent_t next = (iter + 1 < done) ? evs[(size_t)(iter + 1) * per] : E_end;
+                (void)record(iter, h[(size_t)iter * 4], h[(size_t)iter * 4 + 1], ev_ms(E[0], next),
+                             E, nev);
             }
         }
     }

@@ -161,9 +161,13 @@ int dyn_gpu_mttkrp(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_out,
 #define DYN_GPU_NOALLOC_SCRATCH_BYTES (64u << 20)
 int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_dev,
                            int ld);
-/* Every Yhat_n from one snapshot of the factors in one tensor pass
- * (Jacobi all-modes).  Yhat_out may be NULL, or an array of N host
- * pointers (entries may be NULL). */
+/* Every Yhat_n from one snapshot of the factors (Jacobi all-modes
+ * semantics): one segmented spMTTKRP launch per mode over that mode's sorted
+ * copy, all reading the same factors.  Not a single tensor stream: a stream
+ * over one copy would have to scatter every other mode's contributions with
+ * atomics, which measure 30-80x slower than the gathers they replace on this
+ * GPU.  Yhat_out may be NULL, or an array of N host pointers (entries may be
+ * NULL). */
 int dyn_gpu_mttkrp_all(dyn_gpu_ctx* ctx, int precision, float* const* Yhat_out,
                        int ld);
 /* G = Y_mode^T Y_mode (R x R, row-major, FP64 result of FP32 MFMA partials). */

@@ -183,10 +183,19 @@ int cpals_t(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
     double fit_prev = 0.0, fit = 0.0;
     int done = 0;
     std::vector<std::vector<T>> Yb(o->bf16_gather ? N : 0);
+    // the engine's mixed mode gathers the mode whose Yhat carries the fit
+    // (Gauss-Seidel: the last mode; Jacobi: the smallest mode) in FP32, so
+    // the reported fit is that of the FP32 model
+    int fit_mode = N - 1;
+    if (o->jacobi) {
+        fit_mode = 0;
+        for (int m = 1; m < N; ++m)
+            if (dims[m] < dims[fit_mode]) fit_mode = m;
+    }
     auto mttkrp = [&](int n) {
         for (int w = 0; w < N; ++w) {
             Yp[w] = Y[w].data();
-            if (o->bf16_gather && w != n) {
+            if (o->bf16_gather && w != n && n != fit_mode) {
                 Yb[w].resize(Y[w].size());
                 for (size_t e = 0; e < Y[w].size(); ++e) Yb[w][e] = (T)bf16_round((float)Y[w][e]);
                 Yp[w] = Yb[w].data();

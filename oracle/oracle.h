@@ -42,8 +42,10 @@ typedef struct orc_cp_opts {
     int    stale_jacobi_fit;   /* 1: Jacobi fit from the Yhat identity as the reference
                                   prints it (not the true fit); 0: true fit */
     int    bf16_gather;        /* 1: the spMTTKRP reads factor rows rounded to bfloat16
-                                  (round to nearest even of the FP32 value): the
-                                  engine's mixed-precision mode */
+                                  (round to nearest even of the FP32 value), except
+                                  for the mode whose Yhat carries the fit (Gauss-
+                                  Seidel: the last; Jacobi: the smallest), which
+                                  reads FP32 rows: the engine's mixed mode */
 } orc_cp_opts;
 
 int orc_mttkrp(int N, uint64_t nnz, const uint32_t* dims, const float* vals,
