@@ -281,22 +281,37 @@ def completion_leg(wl, warmup, steps, threads):
     loss + adjoint kernel (every mode's gradient per step)."""
     import torch
     from dynasor_plus_amd import GpuContext, host
-    from dynasor_plus_amd.autograd import fit_completion
+    from dynasor_plus_amd.autograd import completion_loss_and_grad
     T, _, _ = host.generate(wl["dims"], wl["nnz"], seed=wl["seed"], coords="uniform",
                             values="bernoulli", planted_rank=wl["planted_rank"],
                             planted_factors="normal", planted_var=wl["planted_var"],
                             threads=threads)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    A = [(0.1 * torch.randn(int(d), wl["rank"], generator=g)).cuda().requires_grad_(True)
+         for d in T.dims]
+    opt = torch.optim.Adam(A, lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    losses = []
+
+    def step():
+        val, grads = completion_loss_and_grad(eng, A, "bernoulli")
+        for a, gr in zip(A, grads):
+            a.grad = gr.contiguous()
+        opt.step()
+        losses.append(val)
+
     with GpuContext(0) as eng:
         eng.upload(T)
         eng.remap_all()
-        fit_completion(eng, T.dims, rank=wl["rank"], steps=warmup, seed=0)
+        for _ in range(warmup):
+            step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, losses = fit_completion(eng, T.dims, rank=wl["rank"], steps=steps, seed=0)
+        for _ in range(steps):
+            step()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
     return {"workload": wl["desc"], "s_per_step": el / steps, "loss_first": float(losses[0]),
-            "loss_last": float(losses[-1])}
+            "loss_last": float(losses[-1]), "steps": steps, "warmup": warmup}
 
 
 def main():

@@ -78,6 +78,9 @@ SIGNATURES = {
     "dyn_gpu_remap": (C.c_int, [_vp, C.c_int, C.c_int, _u64p, _u32p]),
     "dyn_gpu_remap_all": (C.c_int, [_vp]),
     "dyn_gpu_get_copy": (C.c_int, [_vp, C.c_int, _f32p, C.POINTER(_u32p)]),
+    "dyn_gpu_save_processed": (C.c_int, [_vp, C.c_char_p]),
+    "dyn_gpu_load_processed": (C.c_int, [_vp, C.c_char_p]),
+    "dyn_gpu_processed_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), _u64p, _u32p, _u32p]),
     "dyn_gpu_set_factors": (C.c_int, [_vp, C.c_int, C.POINTER(_f32p), C.c_int]),
     "dyn_gpu_get_factors": (C.c_int, [_vp, C.c_int, _f32p, C.c_int]),
     "dyn_gpu_get_lambda": (C.c_int, [_vp, _f64p]),

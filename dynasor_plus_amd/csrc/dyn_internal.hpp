@@ -196,6 +196,13 @@ void lds_limit_once(dyn_gpu_ctx* c, std::atomic<uint64_t>* done, const void* fn,
 
 // building blocks implemented in the .hip units -----------------------------
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary);
+// Columns and row pointer of copy `mode` from cp.perm and the sorted keys.
+int materialize_copy(dyn_gpu_ctx* c, int mode, const uint32_t* keys);
+// Work tables from the row pointer (checked monotone and spanning nnz).
+int finish_copy(dyn_gpu_ctx* c, int mode);
+// Copy `mode` from a stored permutation (host), checked to be a permutation
+// that sorts idx[mode]; no sort runs.
+int restore_copy(dyn_gpu_ctx* c, int mode, int secondary, const uint32_t* perm_host);
 void free_copy(dyn_gpu_ctx* c, int mode);
 // Reorders copy `mode` so that in every row the nonzeros with relabelled
 // mode-w index < H precede the others (stable within both classes) and

@@ -42,6 +42,7 @@ constexpr int kBufDword3 = 0x00020000;   // raw buffer resource word 3 on gfx9-f
 // read once), so the stream does not push gathered factor rows out of L2
 constexpr int kStreamPolicy = 2;
 constexpr int kQuads = kBlock / 4;            // quads per 256-thread block
+constexpr uint32_t kMaxChunks = 2048;         // partials of the two-stage FP64 sums
 
 struct SegArgs {
     const float*    vals;         // copy values, or the override vector
@@ -930,7 +931,7 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     const size_t pbytes = seg_part_bytes(c, mode, Rp);
     const size_t nloss = (size_t)cp.npieces + cp.nseg;
     const size_t lbytes = loss_slots ? ((std::max<size_t>(nloss, 1) * sizeof(double) + 255) & ~size_t(255)) : 0;
-    DYN_CHECK(scratch2(c, pbytes + lbytes + 256, &sp));
+    DYN_CHECK(scratch2(c, pbytes + lbytes + (1 + kMaxChunks) * sizeof(double) + 256, &sp));
     a->part = (float*)sp;
     a->loss_part = loss_slots ? (double*)((char*)sp + pbytes) : nullptr;
     a->loss_dev = (double*)((char*)sp + pbytes + lbytes);
@@ -1108,6 +1109,24 @@ __global__ __launch_bounds__(kBlock) void k_forward(FwdArgs a) {
     }
 }
 
+// Block b sums part[b * chunk, min((b + 1) * chunk, n)) in a fixed order
+// (thread-strided, then the threads in order) into out[b].
+__global__ __launch_bounds__(kBlock) void k_sum_chunks(const double* __restrict__ part, uint32_t n,
+                                                       uint32_t chunk, double* __restrict__ out) {
+    __shared__ double red[kBlock];
+    const uint64_t b0 = (uint64_t)blockIdx.x * chunk;
+    const uint64_t e = std::min<uint64_t>(b0 + chunk, n);
+    double s = 0.0;
+    for (uint64_t i = b0 + threadIdx.x; i < e; i += kBlock) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kBlock; ++i) t += red[i];
+        out[blockIdx.x] = t;
+    }
+}
+
 __global__ void k_sum_parts(const double* __restrict__ part, uint32_t n, double* __restrict__ out) {
     __shared__ double red[kBlock];
     double s = 0.0;
@@ -1119,6 +1138,17 @@ __global__ void k_sum_parts(const double* __restrict__ part, uint32_t n, double*
         for (int i = 0; i < kBlock; ++i) t += red[i];
         *out = t;
     }
+}
+
+// Deterministic FP64 sum of n device values into *out: per-chunk partials
+// (one workgroup each, up to kMaxChunks of them) then one workgroup over the
+// partials.  `tmp` holds at least kMaxChunks doubles and must not alias part.
+static int sum_doubles(dyn_gpu_ctx* c, const double* part, uint32_t n, double* tmp, double* out) {
+    const uint32_t chunk = std::max<uint32_t>(kBlock * 8, (n + kMaxChunks - 1) / kMaxChunks);
+    const uint32_t nb = std::max<uint32_t>(1, (n + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nb), dim3(kBlock), 0, c->stream, part, n, chunk, tmp);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, tmp, nb, out);
+    return check_launch(c, "sum_doubles");
 }
 
 template <int RP>
@@ -1200,9 +1230,7 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         DYN_CHECK(launch_heavy(c, Rp, c->copy[mode], a.part, a.nwaves, grad[mode]));
         if (want_loss) {
             double* d = a.loss_dev;
-            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, c->stream, a.loss_part,
-                               a.npieces + a.nseg, d);
-            DYN_CHECK(check_launch(c, "sum_parts"));
+            DYN_CHECK(sum_doubles(c, a.loss_part, a.npieces + a.nseg, d + 1, d));
             DYN_HIP(c, hipMemcpyAsync(loss_out, d, sizeof(double), hipMemcpyDeviceToHost,
                                       c->stream));
             DYN_HIP(c, hipStreamSynchronize(c->stream));

@@ -269,6 +269,34 @@ __global__ void k_relabel_rows(const float* __restrict__ src, const uint32_t* __
     }
 }
 
+// bad[0] += entries of perm that are out of range or repeated
+__global__ void k_perm_check(const uint32_t* __restrict__ perm, uint32_t n,
+                             uint32_t* __restrict__ bits, unsigned long long* bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long local = 0;
+    for (; i < n; i += stride) {
+        const uint32_t p = perm[i];
+        if (p >= n) {
+            ++local;
+            continue;
+        }
+        const uint32_t m = 1u << (p & 31u);
+        if (atomicOr(&bits[p >> 5], m) & m) ++local;
+    }
+    if (local) atomicAdd(bad, local);
+}
+
+// *bad += positions k with keys[k] < keys[k - 1]
+__global__ void k_count_descents(const uint32_t* __restrict__ keys, uint32_t n,
+                                 unsigned long long* bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long local = 0;
+    for (; i < n; i += stride) local += (keys[i] < keys[i - 1]) ? 1ull : 0ull;
+    if (local) atomicAdd(bad, local);
+}
+
 static int bits_for(uint32_t dim) {
     int b = 0;
     while (b < 32 && ((uint64_t)1 << b) < dim) ++b;
@@ -642,27 +670,109 @@ int remap_mode(dyn_gpu_ctx* c, int mode, int secondary) {
         if (perm != cp.perm)
             DYN_HIP(c, hipMemcpyAsync(cp.perm, perm, (size_t)n * 4, hipMemcpyDeviceToDevice,
                                       c->stream));
-        hipLaunchKernelGGL(k_gather_f32, dim3(g), dim3(kBlock), 0, c->stream, c->vals, cp.perm,
-                           cp.vals, n);
-        for (int w = 0; w < c->N; ++w)
-            if (w != mode)
-                hipLaunchKernelGGL(k_gather_relabel, dim3(g), dim3(kBlock), 0, c->stream,
-                                   c->idx[w], cp.perm, c->relab[w], cp.idx[w], n);
-        hipLaunchKernelGGL(k_rowptr, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
-                           keys, n, dim, cp.rowptr);
-        DYN_CHECK(check_launch(c, "remap gather/rowptr"));
+        DYN_CHECK(materialize_copy(c, mode, keys));
     } else {
         hipLaunchKernelGGL(k_rowptr, dim3(grid_for((uint64_t)dim + 1)), dim3(kBlock), 0,
                            c->stream, kA, 0u, dim, cp.rowptr);
         DYN_CHECK(check_launch(c, "remap empty"));
     }
-    std::vector<uint32_t> rp((size_t)dim + 1);
+    DYN_CHECK(finish_copy(c, mode));
+    cp.secondary = secondary;
+    return DYN_OK;
+}
+
+// The columns and row pointer of copy `mode` from its permutation (cp.perm)
+// and the sorted keys (idx[mode] in sorted order).
+int materialize_copy(dyn_gpu_ctx* c, int mode, const uint32_t* keys) {
+    ModeCopy& cp = c->copy[mode];
+    const uint32_t n = (uint32_t)c->nnz;
+    const unsigned g = grid_for(n);
+    hipLaunchKernelGGL(k_gather_f32, dim3(g), dim3(kBlock), 0, c->stream, c->vals, cp.perm,
+                       cp.vals, n);
+    for (int w = 0; w < c->N; ++w)
+        if (w != mode)
+            hipLaunchKernelGGL(k_gather_relabel, dim3(g), dim3(kBlock), 0, c->stream, c->idx[w],
+                               cp.perm, c->relab[w], cp.idx[w], n);
+    hipLaunchKernelGGL(k_rowptr, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
+                       keys, n, c->dims[mode], cp.rowptr);
+    return check_launch(c, "remap gather/rowptr");
+}
+
+// Work tables of copy `mode` from its row pointer; the copy is then usable.
+int finish_copy(dyn_gpu_ctx* c, int mode) {
+    ModeCopy& cp = c->copy[mode];
+    std::vector<uint32_t> rp((size_t)c->dims[mode] + 1);
     DYN_HIP(c, hipMemcpyAsync(rp.data(), cp.rowptr, rp.size() * 4, hipMemcpyDeviceToHost,
                               c->stream));
     // the scratch buffers are reused by the next call: finish before returning
     DYN_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t r = 0; r + 1 < rp.size(); ++r)
+        if (rp[r] > rp[r + 1] || rp[r + 1] > c->nnz)
+            return set_err(c, DYN_ERR_INVALID, "mode %d row pointer is not monotone", mode);
+    if (rp[0] != 0 || rp.back() != c->nnz)
+        return set_err(c, DYN_ERR_INVALID, "mode %d row pointer does not span the tensor", mode);
     DYN_CHECK(build_segments(c, cp, rp));
     cp.built = true;
+    return DYN_OK;
+}
+
+// Copy `mode` rebuilt from a stored permutation (perm[k] = input position of
+// sorted position k): checks that perm is a permutation whose keys are in
+// non-decreasing order, then gathers the columns as remap_mode does.
+int restore_copy(dyn_gpu_ctx* c, int mode, int secondary, const uint32_t* perm_host) {
+    DYN_CHECK(ensure_relabel(c));
+    free_copy(c, mode);
+    ModeCopy& cp = c->copy[mode];
+    const uint32_t n = (uint32_t)c->nnz;
+    const uint32_t dim = c->dims[mode];
+    DYN_CHECK(dalloc_t(c, &cp.rowptr, (size_t)dim + 1));
+    DYN_CHECK(dalloc_t(c, &cp.perm, std::max<uint32_t>(n, 1)));
+    DYN_CHECK(dalloc_t(c, &cp.vals, std::max<uint32_t>(n, 1)));
+    for (int w = 0; w < c->N; ++w)
+        if (w != mode) DYN_CHECK(dalloc_t(c, &cp.idx[w], std::max<uint32_t>(n, 1)));
+    if (n) {
+        DYN_HIP(c, hipMemcpyAsync(cp.perm, perm_host, (size_t)n * 4, hipMemcpyHostToDevice,
+                                  c->stream));
+        const size_t words = ((size_t)n + 31) / 32;
+        void* sp = nullptr;
+        DYN_CHECK(scratch(c, (size_t)n * 4 + words * 4 + 256, &sp));
+        uint32_t* keys = (uint32_t*)sp;
+        uint32_t* bits = keys + (((size_t)n + 63) & ~size_t(63));
+        void* sp2 = nullptr;
+        DYN_CHECK(scratch2(c, 2 * sizeof(unsigned long long), &sp2));
+        unsigned long long* bad = (unsigned long long*)sp2;
+        const unsigned g = grid_for(n);
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(words)), dim3(kBlock), 0, c->stream, bits,
+                           words);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kBlock), 0, c->stream, (uint32_t*)bad,
+                           (size_t)4);
+        // every entry in range and marked once: a permutation
+        hipLaunchKernelGGL(k_perm_check, dim3(g), dim3(kBlock), 0, c->stream, cp.perm, n, bits,
+                           bad);
+        DYN_CHECK(check_launch(c, "perm_check"));
+        unsigned long long hb[2] = {0, 0};
+        DYN_HIP(c, hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+        DYN_HIP(c, hipStreamSynchronize(c->stream));
+        if (hb[0])
+            return set_err(c, DYN_ERR_INVALID, "stored mode-%d permutation is not a permutation",
+                           mode);
+        hipLaunchKernelGGL(k_gather_u32, dim3(g), dim3(kBlock), 0, c->stream, c->idx[mode],
+                           cp.perm, keys, n);
+        hipLaunchKernelGGL(k_count_descents, dim3(g), dim3(kBlock), 0, c->stream, keys, n,
+                           bad + 1);
+        DYN_CHECK(check_launch(c, "count_descents"));
+        DYN_HIP(c, hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+        DYN_HIP(c, hipStreamSynchronize(c->stream));
+        if (hb[1])
+            return set_err(c, DYN_ERR_INVALID,
+                           "stored mode-%d permutation does not sort the mode's indices", mode);
+        DYN_CHECK(materialize_copy(c, mode, keys));
+    } else {
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((uint64_t)dim + 1)), dim3(kBlock), 0,
+                           c->stream, cp.rowptr, (size_t)dim + 1);
+        DYN_CHECK(check_launch(c, "restore empty"));
+    }
+    DYN_CHECK(finish_copy(c, mode));
     cp.secondary = secondary;
     return DYN_OK;
 }

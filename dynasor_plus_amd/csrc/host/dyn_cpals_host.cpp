@@ -73,8 +73,33 @@ CpResult cpals(Tensor& T, FactorMatrices& F, const CpOptions& opt) {
     };
     int rc = dyn_gpu_create(opt.device, &ctx.h);
     if (rc != DYN_OK) return fail("dyn_gpu_create", rc);
-    rc = dyn_gpu_upload_coo(ctx.h, N, T.nnz, T.mode_size, T.vals, T.idx_buf);
-    if (rc != DYN_OK) return fail("dyn_gpu_upload_coo", rc);
+    if (!opt.processed_path.empty()) {
+        // runtime-only: the device state comes from the processed file, which
+        // must hold the tensor T describes
+        int fN = 0;
+        uint64_t fnnz = 0;
+        uint32_t fdims[DYN_MAX_MODES] = {0}, mask = 0;
+        rc = dyn_gpu_processed_info(opt.processed_path.c_str(), &fN, &fnnz, fdims, &mask);
+        bool same = rc == DYN_OK && fN == N && fnnz == T.nnz;
+        for (int n = 0; same && n < N; ++n) same = fdims[n] == T.mode_size[n];
+        if (!same) {
+            std::fprintf(stderr, "[cpals] '%s' does not hold this tensor\n",
+                         opt.processed_path.c_str());
+            return CpResult();
+        }
+        rc = dyn_gpu_load_processed(ctx.h, opt.processed_path.c_str());
+        if (rc != DYN_OK) return fail("dyn_gpu_load_processed", rc);
+    } else {
+        rc = dyn_gpu_upload_coo(ctx.h, N, T.nnz, T.mode_size, T.vals, T.idx_buf);
+        if (rc != DYN_OK) return fail("dyn_gpu_upload_coo", rc);
+    }
+    if (!opt.save_processed_path.empty()) {
+        rc = dyn_gpu_remap_all(ctx.h);
+        if (rc == DYN_OK) rc = dyn_gpu_save_processed(ctx.h, opt.save_processed_path.c_str());
+        if (rc != DYN_OK) return fail("dyn_gpu_save_processed", rc);
+        if (opt.verbose)
+            std::printf("[io] wrote processed state to %s\n", opt.save_processed_path.c_str());
+    }
     rc = dyn_gpu_set_factors(ctx.h, R, F.Y.data(), Rp);
     if (rc != DYN_OK) return fail("dyn_gpu_set_factors", rc);
 

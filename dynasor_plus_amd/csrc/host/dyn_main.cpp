@@ -6,10 +6,11 @@
 //                              --verify / --dump / --dump-bin / --all-modes, so
 //                              the reference's scripts/run_tests.sh can drive it
 // CPU-layout flags of the reference (--layout, --ncopy-*, --shard-size, --jit,
-// --ooc*, --threads) are accepted and ignored; --runtime / --save-processed
-// are rejected (no processed-state cache yet).
+// --ooc*, --threads) are accepted and ignored; --save-processed / --runtime
+// write and read the device's processed state (dynasor_dnp.h).
 #include "dynasor_common.h"
 #include "dynasor_cpals.h"
+#include "dynasor_dnp.h"
 #include "dynasor_gpu.h"
 #include "dynasor_host.h"
 
@@ -46,6 +47,8 @@ struct Args {
     bool all_modes = false;
     bool uniform_init = false;
     int device = 0;
+    std::string save_processed;
+    bool runtime_only = false;
 };
 
 void usage(const char* a0) {
@@ -68,7 +71,11 @@ void usage(const char* a0) {
         "  --cp-init-load PREFIX  start from <PREFIX>.cp_init_fm<n>.bin\n"
         "  --cp-init-uniform      U[0,1) initial factors instead of N(0,1)\n"
         "  --precision P          fp32 | bf16 gathered factors (env DYN_CP_PRECISION)\n"
-        "  --device D             GPU ordinal (default 0)\n",
+        "  --device D             GPU ordinal (default 0)\n"
+        "  --save-processed P     write the processed device state (tensor + per-mode\n"
+        "                         permutations) to P after the copies are built\n"
+        "  --runtime              <tensor> is a processed-state file: load it and skip\n"
+        "                         the device sort (needs --decompose cpals)\n",
         a0);
 }
 
@@ -109,10 +116,10 @@ bool parse(int argc, char** argv, Args& a) {
                  k == "--ooc-chunk") {
             (void)need(k.c_str());   // CPU layout knobs: no device counterpart
         } else if (k == "--jit") {
-        } else if (k == "--runtime" || k == "--save-processed") {
-            std::fprintf(stderr, "%s: processed-state files are not supported by the device engine\n",
-                         k.c_str());
-            return false;
+        } else if (k == "--save-processed") {
+            a.save_processed = need("--save-processed");
+        } else if (k == "--runtime") {
+            a.runtime_only = true;
         } else if (k == "-h" || k == "--help") {
             return false;
         } else {
@@ -242,7 +249,17 @@ int main(int argc, char** argv) {
     std::printf("=== Dynasor+ MI355X engine (backend: HIP gfx950) ===\n");
     std::unique_ptr<Tensor> Tp(new Tensor());
     Tensor& T = *Tp;
-    if (!load_coo_tensor(args.tensor_path, T, !args.no_cache)) {
+    if (args.runtime_only) {
+        // src/main.cpp:362-427: a processed file replaces load + preprocessing
+        if (args.verify || args.decompose != "cpals") {
+            std::fprintf(stderr, "[main] --runtime needs --decompose cpals and excludes --verify\n");
+            return 7;
+        }
+        if (!load_dnp_header(args.tensor_path, T)) return 2;
+        std::printf("[runtime] loading processed state: %s\n          modes=%d  nnz=%llu  "
+                    "layout=device-copies\n",
+                    args.tensor_path.c_str(), T.num_modes, (unsigned long long)T.nnz);
+    } else if (!load_coo_tensor(args.tensor_path, T, !args.no_cache)) {
         std::fprintf(stderr, "failed to load tensor '%s'\n", args.tensor_path.c_str());
         return 2;
     }
@@ -260,6 +277,8 @@ int main(int argc, char** argv) {
         opt.init_load_prefix = args.cp_init_load;
         opt.uniform_init = args.uniform_init;
         opt.device = args.device;
+        opt.save_processed_path = args.save_processed;
+        if (args.runtime_only) opt.processed_path = args.tensor_path;
         ComputePlan cp = dyn_build_compute_plan(T, F.rank_padded, args.threads, "auto", "auto",
                                                 args.cp_kernel);
         if (!args.precision.empty()) {
@@ -311,6 +330,10 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "[main] unknown --decompose algorithm: %s (supported: cpals)\n",
                      args.decompose.c_str());
         return 5;
+    }
+    if (!args.save_processed.empty()) {
+        if (!save_dnp_tensor(args.save_processed, T, args.threads, args.device)) return 2;
+        std::printf("[io] wrote processed state to %s\n", args.save_processed.c_str());
     }
     return mttkrp_mode_main(args, T);
 }

@@ -48,6 +48,11 @@ struct CpOptions {
     int         device = 0;
     bool        uniform_init = false;  // U[0,1) (std::mt19937_64(seed)) instead of N(0,1)
     std::string init_load_prefix;      // read <prefix>.cp_init_fm<n>.bin instead of seeding
+    // processed state (dynasor_dnp.h): load the device tensor and its copies
+    // from this file instead of uploading T and sorting (T needs only its
+    // header), and/or write the processed state after the copies are built
+    std::string processed_path;
+    std::string save_processed_path;
 };
 
 struct CpResult {

@@ -132,6 +132,20 @@ class GpuContext:
         self._check(self.lib.dyn_gpu_get_copy(self.h, mode, _f32ptr(vals), arr), "get_copy")
         return vals[: self.nnz], {w: cols[w][: self.nnz] for w in range(self.N) if w != mode}
 
+    # -- processed state ---------------------------------------------------
+    def save_processed(self, path) -> None:
+        """Write the tensor and every built copy's permutation (dyn_gpu_save_processed)."""
+        self._check(self.lib.dyn_gpu_save_processed(self.h, str(path).encode()),
+                    "save_processed")
+
+    def load_processed(self, path) -> None:
+        """Replace the tensor with a processed-state file's; its copies are
+        gathered through the stored permutations (no sort)."""
+        self._check(self.lib.dyn_gpu_load_processed(self.h, str(path).encode()),
+                    "load_processed")
+        N, nnz, dims, _ = processed_info(path)
+        self.N, self.dims, self.nnz, self.R = N, dims, nnz, 0
+
     # -- factors -----------------------------------------------------------
     def set_factors(self, Y: list | None, rank: int) -> None:
         if Y is None:
@@ -215,6 +229,21 @@ class GpuContext:
                         mttkrp_time_s=r.mttkrp_time_s, linalg_time_s=r.linalg_time_s,
                         remap_time_s=r.remap_time_s, fit_history=list(hist[:k]),
                         iter_time_s=list(itim[:k]), lambda_=self.get_lambda())
+
+
+def processed_info(path):
+    """(num_modes, nnz, dims, copies_mask) of a processed-state file; raises
+    ValueError if `path` is not one (needs no device)."""
+    lib = L.load_library()
+    n = C.c_int(0)
+    nnz = C.c_uint64(0)
+    dims = (C.c_uint32 * L.MAX_MODES)()
+    mask = C.c_uint32(0)
+    rc = lib.dyn_gpu_processed_info(str(path).encode(), C.byref(n), C.byref(nnz), dims,
+                                    C.byref(mask))
+    if rc != L.DYN_OK:
+        raise ValueError(f"{path} is not a processed-state file (status {rc})")
+    return n.value, int(nnz.value), tuple(int(dims[i]) for i in range(n.value)), int(mask.value)
 
 
 # ---------------------------------------------------------------------------

@@ -130,6 +130,24 @@ int dyn_gpu_remap_all(dyn_gpu_ctx* ctx);
 int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
                      uint32_t* const* idx_out);
 
+/* ---- processed state (replaces save_dnp_tensor / load_dnp_tensor,
+ *      include/dynasor_dnp.h:110-145) ------------------------------------ */
+/* Write the uploaded tensor and the permutation of every built copy to
+ * `path` ("DGP1" format: header, COO in input order, one u32 permutation
+ * per copy), through `path`.tmp and a rename. */
+int dyn_gpu_save_processed(dyn_gpu_ctx* ctx, const char* path);
+/* Replace the context's tensor with the file's and rebuild the stored copies
+ * by gathering through their permutations (no sort).  Every index is checked
+ * against its mode size and every permutation must be a permutation of
+ * [0, nnz) that orders its mode's indices, else DYN_ERR_INVALID.  Factors are
+ * dropped, as by dyn_gpu_upload_coo. */
+int dyn_gpu_load_processed(dyn_gpu_ctx* ctx, const char* path);
+/* Header of a processed-state file (no device needed): mode count, nnz,
+ * dims[num_modes] and the mask of stored copies; DYN_ERR_INVALID for a file
+ * that is not one, or whose size does not match its header. */
+int dyn_gpu_processed_info(const char* path, int* num_modes, uint64_t* nnz, uint32_t* dims,
+                           uint32_t* copies_mask);
+
 /* ---- factors --------------------------------------------------------- */
 /* Allocate factor storage for rank R (device rows padded to a power of two
  * >= 16) and copy Y[n] (I_n x R, leading dimension ld) in; Y may be NULL to

@@ -189,7 +189,11 @@ def test_dyfm_roundtrip(tmp_path):
 def test_cli_usage_and_errors(tmp_path, d37):
     assert subprocess.run([str(CLI)], capture_output=True).returncode == 1
     assert subprocess.run([str(CLI), str(d37), "--bogus"], capture_output=True).returncode == 1
-    assert subprocess.run([str(CLI), str(d37), "--runtime"], capture_output=True).returncode == 1
+    # --runtime needs --decompose cpals (exit 7, as the reference's conflicts), and
+    # a processed-state file (a .tns is not one: exit 2)
+    assert subprocess.run([str(CLI), str(d37), "--runtime"], capture_output=True).returncode == 7
+    assert subprocess.run([str(CLI), str(d37), "--runtime", "--decompose", "cpals"],
+                          capture_output=True).returncode == 2
     assert subprocess.run([str(CLI), str(tmp_path / "missing.tns")],
                           capture_output=True).returncode == 2
     r = subprocess.run([str(CLI), str(d37), "--decompose", "tucker", "--no-cache"],
