@@ -224,7 +224,7 @@ def kernel_figures(dims, nnz, R, jacobi, steps, launches, kms, bf):
             "bytes_per_launch": total / launches if launches else None}
 
 
-def cpals_leg(name, wl, warmup, steps, threads, want_cpu):
+def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True):
     """Plain FP32 and mixed BF16-gather CP-ALS on one workload: s/iter, fits,
     equal-fit check and the spMTTKRP roofline figures of each."""
     from dynasor_plus_amd import GpuContext, host, init_factors
@@ -247,7 +247,8 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu):
         ctx.remap_all()
         out["remap_s"] = time.time() - t0
         runs = {}
-        for label, prec, bf in (("mixed", L.DYN_PREC_BF16, 2), ("plain", L.DYN_PREC_FP32, 4)):
+        legs = (("mixed", L.DYN_PREC_BF16, 2), ("plain", L.DYN_PREC_FP32, 4))
+        for label, prec, bf in (legs if mixed else legs[1:]):
             res, elapsed, launches, kms = timed_run(ctx, Y0, R, kern, prec, warmup, steps)
             k = kernel_figures(dims, T.nnz, R, jacobi, steps, launches, kms, bf)
             k["s_per_iter"] = elapsed / steps
@@ -261,6 +262,8 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu):
             log(f"{name} {label}: {k['s_per_iter'] * 1e3:.3f} ms/iter, spMTTKRP "
                 f"{k['ms_per_launch']:.3f} ms/launch -> {k['GBps']:.0f} GB/s, "
                 f"{k['GFLOPs']:.0f} GFLOP/s, final fit {res.final_fit}")
+    if not mixed:
+        runs["mixed"] = runs["plain"]
     fm, fp = runs["mixed"]["final_fit"], runs["plain"]["final_fit"]
     equal = math.isfinite(fm) and math.isfinite(fp) and abs(fm - fp) <= EQUAL_FIT_TOL
     out.update(runs)
@@ -323,6 +326,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 3- and 4-mode Gauss-Seidel comparison legs")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="plain FP32 gathers only (the line then reports that run)")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
@@ -342,7 +347,8 @@ def main():
                                                                       threads)}), flush=True)
         return
 
-    head = cpals_leg(args.workload, wl, args.warmup, steps, threads, not args.no_cpu)
+    head = cpals_leg(args.workload, wl, args.warmup, steps, threads, not args.no_cpu,
+                     mixed=not args.no_mixed)
     extra = {}
     if args.workload == "c4" and not args.no_extra:
         for name in ("c2", "c3"):
@@ -368,7 +374,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 gathered operands, f32 accumulate, f64 Cholesky/lambda/fit",
+        "dtype": ("f32 gathered operands" if args.no_mixed else "bf16 gathered operands") +
+                 ", f32 accumulate, f64 Cholesky/lambda/fit",
         "data": "synthetic (engine generator, distinct coordinates); N(0,1) initial factors, seed 42",
         "config": {"workload": head["workload"], "dims": head["dims"], "nnz": head["nnz"],
                    "rank": head["rank"], "kernel": head["kernel"], "parallelism": "single GPU"},

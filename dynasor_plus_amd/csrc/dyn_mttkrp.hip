@@ -580,7 +580,7 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
             }
             // round t: nonzeros 16 g + 4 t + j, held by lane t of the quad
             // a hot-split piece reads operand 1 from LDS (hot) or memory (cold)
-            const bool hot1 = kSplitPath<NO, FT>() && a.pc_hot && a.pc_hot[p] != 0u;
+            const bool hot1 = kSplitPath<NO, FT>() && a.pc_hot[p] != 0u;
             auto do_rounds = [&](auto t0c, auto splitc) {
                 constexpr int T0 = decltype(t0c)::value;
                 constexpr int SP = decltype(splitc)::value;
@@ -816,6 +816,9 @@ static int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
             bytes += (size_t)a.hot[w] * rowb;
         }
     }
+    // a 3-mode launch reads the copy's hot-piece flags (the hot split)
+    if (kSplitPath<NO, FT>() && !a.pc_hot)
+        return set_err(c, DYN_ERR_STATE, "3-mode spMTTKRP launch without a hot split");
     auto fn = k_seg<RP, NO, FT, KIND>;
     static std::atomic<uint64_t> lds_set{0};
     lds_limit_once(c, &lds_set, (const void*)fn, (int)std::min<size_t>(c->max_lds, 160u * 1024u));
