@@ -783,8 +783,15 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         return std::chrono::duration<double>(b - a).count();
     };
     const int N = c->N;
-    const bool jacobi = o->kernel == DYN_CP_JACOBI;
+    bool jacobi = o->kernel == DYN_CP_JACOBI;
     const bool bf = o->precision == DYN_PREC_BF16;
+    if (c->streamed && !jacobi) {
+        // src/dynasor_cpals.cpp:223-231: the streaming driver is all-modes only
+        std::fprintf(stderr,
+                     "[cpals] WARN: the tensor is streamed from host memory; Gauss-Seidel has "
+                     "no streaming driver: running Jacobi (all-modes) instead\n");
+        jacobi = true;
+    }
 
     double* hist = res->fit_history;
     double* itim = res->iter_time_s;
@@ -793,7 +800,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     res->iter_time_s = itim;
 
     auto t_r0 = clk::now();
-    for (int m = 0; m < N; ++m)
+    for (int m = 0; m < N && !c->streamed; ++m)
         if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
     DYN_HIP(c, hipStreamSynchronize(c->stream));
     res->remap_time_s = secs(t_r0, clk::now());
@@ -906,6 +913,15 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         return mttkrp_mode(c, n, f32 ? DYN_PREC_FP32 : o->precision, f32 ? Yin32 : Yin, nullptr,
                            c->Yhat[n], c->Rp);
     };
+    int precs[kMaxModes];
+    for (int m = 0; m < N; ++m) precs[m] = (bf && m != fit_mode) ? DYN_PREC_BF16 : DYN_PREC_FP32;
+    // the spMTTKRPs of the modes in `mask` (Jacobi), in core or streamed
+    auto sweep = [&](uint32_t mask) -> int {
+        if (c->streamed) return stream_sweep(c, mask, precs);
+        for (int m = 0; m < N; ++m)
+            if ((mask >> m) & 1u) DYN_CHECK(mttkrp_n(m));
+        return DYN_OK;
+    };
     // fit of the current device model from Yhat[s_mode] (Jacobi) into `out`
     auto jacobi_fit = [&](double* out) -> int {
         double* ipart = nullptr;
@@ -986,7 +1002,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         (void)hipEventRecord(E[0], c->stream);
         if (jacobi) {
             (void)hipEventRecord(E[1], c->stream);
-            for (int m = 0; m < N && rc == DYN_OK; ++m) rc = mttkrp_n(m);
+            rc = sweep((1u << N) - 1u);
             (void)hipEventRecord(E[2], c->stream);
             if (rc != DYN_OK) break;
             if (iter > 0) {
@@ -1018,7 +1034,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     // Jacobi: the last updated model's fit, from one more spMTTKRP of s_mode
     if (rc == DYN_OK && jacobi && !stopped && done > 0) {
         double* out = deferred ? hist_dev + (size_t)(done - 1) * 4 : c->fitbuf;
-        rc = mttkrp_n(s_mode);
+        rc = sweep(1u << s_mode);
         if (rc == DYN_OK) rc = jacobi_fit(out);
         if (rc == DYN_OK && !deferred) {
             rc = read_fit(done - 1);
@@ -1064,6 +1080,7 @@ static bool rp_ok(int ld) { return ld == 16 || ld == 32 || ld == 64 || ld == 128
 extern "C" int dyn_gpu_forward(dyn_gpu_ctx* c, int use_lambda, float* m_out) {
     if (!c || !m_out) return DYN_ERR_INVALID;
     if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_forward"));
     DYN_HIP(c, hipSetDevice(c->device));
     if (c->nnz == 0) return DYN_OK;
     void* sp = nullptr;
@@ -1082,6 +1099,7 @@ extern "C" int dyn_gpu_adjoint(dyn_gpu_ctx* c, const float* g, int mode, float* 
                                int ld) {
     if (!c || !g || !grad_out) return DYN_ERR_INVALID;
     if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_adjoint"));
     if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
     if (ld < c->R) return set_err(c, DYN_ERR_INVALID, "ld %d < rank %d", ld, c->R);
     DYN_HIP(c, hipSetDevice(c->device));
@@ -1112,6 +1130,7 @@ extern "C" int dyn_gpu_adjoint(dyn_gpu_ctx* c, const float* g, int mode, float* 
 extern "C" int dyn_gpu_forward_dev(dyn_gpu_ctx* c, const float* const* A, int ld, float* m) {
     if (!c || !A || !m) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_forward_dev"));
     if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
     DYN_HIP(c, hipSetDevice(c->device));
     if (c->nnz == 0) return DYN_OK;
@@ -1122,6 +1141,7 @@ extern "C" int dyn_gpu_adjoint_dev(dyn_gpu_ctx* c, const float* g, int mode,
                                    const float* const* A, int ld, float* grad) {
     if (!c || !g || !A || !grad) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_adjoint_dev"));
     if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
     if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);
     DYN_HIP(c, hipSetDevice(c->device));
@@ -1133,6 +1153,7 @@ extern "C" int dyn_gpu_completion_grad_dev(dyn_gpu_ctx* c, int loss, const float
                                            int ld, float* const* grad, double* loss_out) {
     if (!c || !A || !grad) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_completion_grad_dev"));
     if (loss != DYN_LOSS_LS && loss != DYN_LOSS_BERNOULLI)
         return set_err(c, DYN_ERR_INVALID, "bad loss %d", loss);
     if (!rp_ok(ld)) return set_err(c, DYN_ERR_INVALID, "ld must be 16, 32, 64 or 128 (got %d)", ld);

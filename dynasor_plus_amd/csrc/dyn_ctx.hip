@@ -167,7 +167,7 @@ __global__ void k_count_oob(const uint32_t* __restrict__ ix, uint64_t n, uint32_
     if (local) atomicAdd(bad, local);
 }
 
-static void free_factors(dyn_gpu_ctx* c) {
+void free_factors(dyn_gpu_ctx* c) {
     for (int n = 0; n < kMaxModes; ++n) {
         dfree(c, c->Y[n]);
         dfree(c, c->Yg[n]);
@@ -180,6 +180,9 @@ static void free_factors(dyn_gpu_ctx* c) {
         c->Yhat[n] = nullptr;
         c->G[n] = nullptr;
     }
+    dfree(c, c->stream_tmp);
+    c->stream_tmp = nullptr;
+    c->stream_tmp_elems = 0;
     dfree(c, c->lambda);
     dfree(c, c->inv_lambda);
     dfree(c, c->L);
@@ -219,7 +222,7 @@ void free_relabel(dyn_gpu_ctx* c) {
     c->gather_valid = false;
 }
 
-static void free_tensor(dyn_gpu_ctx* c) {
+void free_tensor(dyn_gpu_ctx* c) {
     for (int m = 0; m < kMaxModes; ++m) free_copy(c, m);
     free_relabel(c);
     dfree(c, c->vals);
@@ -232,6 +235,11 @@ static void free_tensor(dyn_gpu_ctx* c) {
     c->N = 0;
     c->nnz = 0;
     c->tnorm2 = -1.0;
+    c->streamed = false;
+    c->host_vals = nullptr;
+    for (int n = 0; n < kMaxModes; ++n) c->host_idx[n] = nullptr;
+    c->total_nnz = 0;
+    c->chunk_nnz = 0;
 }
 
 }  // namespace dyn
@@ -436,6 +444,7 @@ extern "C" int dyn_gpu_remap(dyn_gpu_ctx* c, int mode, int secondary,
                              uint64_t* rowptr_out, uint32_t* perm_out) {
     if (!c) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_remap"));
     if (mode < 0 || mode >= c->N)
         return set_err(c, DYN_ERR_INVALID, "mode %d out of range", mode);
     if (secondary >= c->N || secondary == mode)
@@ -461,6 +470,7 @@ extern "C" int dyn_gpu_remap(dyn_gpu_ctx* c, int mode, int secondary,
 extern "C" int dyn_gpu_remap_all(dyn_gpu_ctx* c) {
     if (!c) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_remap_all"));
     DYN_HIP(c, hipSetDevice(c->device));
     for (int m = 0; m < c->N; ++m) {
         if (c->copy[m].built) continue;
@@ -474,6 +484,7 @@ extern "C" int dyn_gpu_get_copy(dyn_gpu_ctx* c, int mode, float* vals_out,
                                 uint32_t* const* idx_out) {
     if (!c) return DYN_ERR_INVALID;
     if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_get_copy"));
     const ModeCopy& cp = c->copy[mode];
     if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
     if (c->nnz == 0) return DYN_OK;
@@ -612,10 +623,16 @@ extern "C" int dyn_gpu_mttkrp(dyn_gpu_ctx* c, int mode, int prec, float* out, in
         return set_err(c, DYN_ERR_INVALID, "bad precision %d", prec);
     if (out && ld < c->R) return set_err(c, DYN_ERR_INVALID, "ld %d < rank %d", ld, c->R);
     DYN_HIP(c, hipSetDevice(c->device));
-    if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
-    const void* Yin[kMaxModes];
-    DYN_CHECK(gathered_operands(c, prec, Yin));
-    DYN_CHECK(mttkrp_mode(c, mode, prec, Yin, nullptr, c->Yhat[mode], c->Rp));
+    if (c->streamed) {
+        int precs[kMaxModes];
+        for (int m = 0; m < c->N; ++m) precs[m] = prec;
+        DYN_CHECK(stream_sweep(c, 1u << mode, precs));
+    } else {
+        if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
+        const void* Yin[kMaxModes];
+        DYN_CHECK(gathered_operands(c, prec, Yin));
+        DYN_CHECK(mttkrp_mode(c, mode, prec, Yin, nullptr, c->Yhat[mode], c->Rp));
+    }
     if (out) DYN_CHECK(copy_out_rows(c, c->Yhat[mode], c->dims[mode], c->Rp, c->R, out, ld));
     return DYN_OK;
 }
@@ -624,6 +641,7 @@ extern "C" int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* c, int mode, int prec, float*
                                       int ld) {
     if (!c || !Yhat_dev) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_mttkrp_noalloc"));
     if (!c->Y[0]) return set_err(c, DYN_ERR_STATE, "factors not set");
     if (mode < 0 || mode >= c->N) return set_err(c, DYN_ERR_INVALID, "bad mode %d", mode);
     if (prec != DYN_PREC_FP32 && prec != DYN_PREC_BF16)
@@ -658,13 +676,19 @@ extern "C" int dyn_gpu_mttkrp_all(dyn_gpu_ctx* c, int prec, float* const* out, i
         return set_err(c, DYN_ERR_INVALID, "bad precision %d", prec);
     if (out && ld < c->R) return set_err(c, DYN_ERR_INVALID, "ld %d < rank %d", ld, c->R);
     DYN_HIP(c, hipSetDevice(c->device));
-    for (int m = 0; m < c->N; ++m)
-        if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
-    const void* Yin[kMaxModes];
-    DYN_CHECK(gathered_operands(c, prec, Yin));
-    // one snapshot of the factors feeds every mode (Jacobi semantics)
-    for (int m = 0; m < c->N; ++m)
-        DYN_CHECK(mttkrp_mode(c, m, prec, Yin, nullptr, c->Yhat[m], c->Rp));
+    if (c->streamed) {
+        int precs[kMaxModes];
+        for (int m = 0; m < c->N; ++m) precs[m] = prec;
+        DYN_CHECK(stream_sweep(c, (1u << c->N) - 1u, precs));
+    } else {
+        for (int m = 0; m < c->N; ++m)
+            if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
+        const void* Yin[kMaxModes];
+        DYN_CHECK(gathered_operands(c, prec, Yin));
+        // one snapshot of the factors feeds every mode (Jacobi semantics)
+        for (int m = 0; m < c->N; ++m)
+            DYN_CHECK(mttkrp_mode(c, m, prec, Yin, nullptr, c->Yhat[m], c->Rp));
+    }
     if (out)
         for (int m = 0; m < c->N; ++m)
             if (out[m])

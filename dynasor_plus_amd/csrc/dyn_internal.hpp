@@ -132,6 +132,18 @@ struct dyn_gpu_ctx {
     int          num_cus = 256;
     size_t       max_lds = 65536;
 
+    // out-of-HBM streaming (dyn_gpu_attach_host_coo): the tensor stays in
+    // caller host memory and passes through the device chunk buffers (vals /
+    // idx, chunk_nnz long) once per sweep; nnz is the whole tensor's count
+    // except inside a sweep, where it is the current chunk's
+    bool         streamed = false;
+    const float* host_vals = nullptr;
+    const uint32_t* host_idx[dyn::kMaxModes] = {nullptr};
+    uint64_t     total_nnz = 0;
+    uint64_t     chunk_nnz = 0;
+    float*       stream_tmp = nullptr;            // one mode's chunk spMTTKRP
+    size_t       stream_tmp_elems = 0;
+
     // device address of this device's fixed partials array (no-allocation
     // entry point), from hipGetSymbolAddress at context creation
     void*        fixed_scratch = nullptr;
@@ -204,6 +216,9 @@ int finish_copy(dyn_gpu_ctx* c, int mode);
 // that sorts idx[mode]; no sort runs.
 int restore_copy(dyn_gpu_ctx* c, int mode, int secondary, const uint32_t* perm_host);
 void free_copy(dyn_gpu_ctx* c, int mode);
+// drop the tensor (copies, relabel, streaming state) / the factor storage
+void free_tensor(dyn_gpu_ctx* c);
+void free_factors(dyn_gpu_ctx* c);
 // Reorders copy `mode` so that in every row the nonzeros with relabelled
 // mode-w index < H precede the others (stable within both classes) and
 // rebuilds its work tables with hot/cold pieces.  No-op when already split
@@ -229,6 +244,13 @@ bool split_ready(const dyn_gpu_ctx* c, int mode, int Rp);
 int adjoint_mode(dyn_gpu_ctx* c, int mode, const float* const* A, int Rp, const float* g,
                  float* out);
 
+// Streamed sweep: Yhat[n] for every mode n in `mask` over all chunks of an
+// attached host tensor, mode n gathering at precision prec[n]; chunk results
+// are added in chunk order.
+int stream_sweep(dyn_gpu_ctx* c, uint32_t mask, const int* prec);
+// DYN_ERR_STATE for entry points that need the whole tensor on the device
+int reject_streamed(dyn_gpu_ctx* c, const char* what);
+
 int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp,
                  const double* lambda_or_null, float* m_out, const float* vals_dot,
                  double* dot_out_dev);
@@ -238,6 +260,11 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
 int tensor_norm2(dyn_gpu_ctx* c, double* out);
 int gram_mode(dyn_gpu_ctx* c, int mode);
 int ensure_relabel(dyn_gpu_ctx* c);
+// histogram of ix[0, n) over [0, dim) added into cnt (device)
+int count_rows(dyn_gpu_ctx* c, const uint32_t* ix, uint64_t n, uint32_t dim, uint32_t* cnt);
+int zero_u32(dyn_gpu_ctx* c, uint32_t* p, size_t n);
+// relab[w] (allocated if needed) and its host inverse from per-row counts
+int relabel_from_counts(dyn_gpu_ctx* c, int w, const uint32_t* cnt);
 // Yg / Ygb of every mode from Y (after set_factors)
 int ensure_gather(dyn_gpu_ctx* c);
 // dst32[relab[i]] = src[i] (and dstbf, if non-null), rows x Rp

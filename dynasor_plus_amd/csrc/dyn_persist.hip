@@ -132,6 +132,7 @@ using namespace dyn;
 extern "C" int dyn_gpu_save_processed(dyn_gpu_ctx* c, const char* path) {
     if (!c || !path) return DYN_ERR_INVALID;
     if (c->N == 0) return set_err(c, DYN_ERR_STATE, "no tensor uploaded");
+    DYN_CHECK(reject_streamed(c, "dyn_gpu_save_processed"));
     DYN_HIP(c, hipSetDevice(c->device));
     DYN_HIP(c, hipStreamSynchronize(c->stream));
     Header h;

@@ -329,6 +329,45 @@ static int radix_passes(dyn_gpu_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1
     return DYN_OK;
 }
 
+int count_rows(dyn_gpu_ctx* c, const uint32_t* ix, uint64_t n, uint32_t dim, uint32_t* cnt) {
+    if (n == 0) return DYN_OK;
+    const uint32_t nblk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 65535) / 65536, 1),
+                                                       (uint64_t)c->num_cus * 4);
+    static std::atomic<uint64_t> lds_set{0};
+    lds_limit_once(c, &lds_set, (const void*)k_bincount, (int)(kHistBins * sizeof(uint32_t)));
+    for (uint32_t lo = 0; lo < dim; lo += kHistBins) {
+        const uint32_t nb = std::min(kHistBins, dim - lo);
+        hipLaunchKernelGGL(k_bincount, dim3(nblk), dim3(kHistBlock), nb * sizeof(uint32_t),
+                           c->stream, ix, (uint32_t)n, lo, nb, cnt);
+        DYN_CHECK(check_launch(c, "bincount"));
+    }
+    return DYN_OK;
+}
+
+int zero_u32(dyn_gpu_ctx* c, uint32_t* p, size_t n) {
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, n);
+    return check_launch(c, "zero_u32");
+}
+
+int relabel_from_counts(dyn_gpu_ctx* c, int w, const uint32_t* cnt) {
+    const uint32_t dim = c->dims[w];
+    if (!c->relab[w]) DYN_CHECK(dalloc_t(c, &c->relab[w], dim));
+    std::vector<uint32_t> h(dim);
+    DYN_HIP(c, hipMemcpyAsync(h.data(), cnt, (size_t)dim * 4, hipMemcpyDeviceToHost, c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<uint32_t>& inv = c->relab_inv_host[w];
+    inv.resize(dim);
+    std::iota(inv.begin(), inv.end(), 0u);
+    std::stable_sort(inv.begin(), inv.end(), [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
+    std::vector<uint32_t> rank(dim);
+    for (uint32_t r = 0; r < dim; ++r) rank[inv[r]] = r;
+    DYN_HIP(c, hipMemcpyAsync(c->relab[w], rank.data(), (size_t)dim * 4, hipMemcpyHostToDevice,
+                              c->stream));
+    DYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DYN_OK;
+}
+
 // Frequency ranks of every mode's indices: rank 0 is the index with the most
 // nonzeros (ties by index).  The ranking only orders factor rows for the
 // gathers; results do not depend on it.
@@ -336,37 +375,12 @@ int ensure_relabel(dyn_gpu_ctx* c) {
     if (c->relabeled) return DYN_OK;
     for (int w = 0; w < c->N; ++w) {
         const uint32_t dim = c->dims[w];
-        DYN_CHECK(dalloc_t(c, &c->relab[w], dim));
         void* sp = nullptr;
         DYN_CHECK(scratch(c, (size_t)dim * sizeof(uint32_t), &sp));
         uint32_t* cnt = (uint32_t*)sp;
-        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(dim)), dim3(kBlock), 0, c->stream, cnt,
-                           (size_t)dim);
-        if (c->nnz) {
-            const uint32_t nblk = (uint32_t)std::min<uint64_t>(
-                std::max<uint64_t>((c->nnz + 65535) / 65536, 1), (uint64_t)c->num_cus * 4);
-            static std::atomic<uint64_t> lds_set{0};
-            lds_limit_once(c, &lds_set, (const void*)k_bincount, (int)(kHistBins * sizeof(uint32_t)));
-            for (uint32_t lo = 0; lo < dim; lo += kHistBins) {
-                const uint32_t nb = std::min(kHistBins, dim - lo);
-                hipLaunchKernelGGL(k_bincount, dim3(nblk), dim3(kHistBlock), nb * sizeof(uint32_t),
-                                   c->stream, c->idx[w], (uint32_t)c->nnz, lo, nb, cnt);
-                DYN_CHECK(check_launch(c, "bincount"));
-            }
-        }
-        std::vector<uint32_t> h(dim);
-        DYN_HIP(c, hipMemcpyAsync(h.data(), cnt, (size_t)dim * 4, hipMemcpyDeviceToHost, c->stream));
-        DYN_HIP(c, hipStreamSynchronize(c->stream));
-        std::vector<uint32_t>& inv = c->relab_inv_host[w];
-        inv.resize(dim);
-        std::iota(inv.begin(), inv.end(), 0u);
-        std::stable_sort(inv.begin(), inv.end(),
-                         [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
-        std::vector<uint32_t> rank(dim);
-        for (uint32_t r = 0; r < dim; ++r) rank[inv[r]] = r;
-        DYN_HIP(c, hipMemcpyAsync(c->relab[w], rank.data(), (size_t)dim * 4,
-                                  hipMemcpyHostToDevice, c->stream));
-        DYN_HIP(c, hipStreamSynchronize(c->stream));
+        DYN_CHECK(zero_u32(c, cnt, dim));
+        DYN_CHECK(count_rows(c, c->idx[w], c->nnz, dim, cnt));
+        DYN_CHECK(relabel_from_counts(c, w, cnt));
     }
     c->relabeled = true;
     return DYN_OK;

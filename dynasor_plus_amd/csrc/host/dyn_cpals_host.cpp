@@ -89,6 +89,10 @@ CpResult cpals(Tensor& T, FactorMatrices& F, const CpOptions& opt) {
         }
         rc = dyn_gpu_load_processed(ctx.h, opt.processed_path.c_str());
         if (rc != DYN_OK) return fail("dyn_gpu_load_processed", rc);
+    } else if (opt.stream_from_host) {
+        rc = dyn_gpu_attach_host_coo(ctx.h, N, T.nnz, T.mode_size, T.vals, T.idx_buf,
+                                     opt.stream_chunk_nnz);
+        if (rc != DYN_OK) return fail("dyn_gpu_attach_host_coo", rc);
     } else {
         rc = dyn_gpu_upload_coo(ctx.h, N, T.nnz, T.mode_size, T.vals, T.idx_buf);
         if (rc != DYN_OK) return fail("dyn_gpu_upload_coo", rc);

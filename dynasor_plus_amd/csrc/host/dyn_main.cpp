@@ -49,6 +49,9 @@ struct Args {
     int device = 0;
     std::string save_processed;
     bool runtime_only = false;
+    std::string ooc = "auto";
+    std::string layout = "auto";
+    uint64_t ooc_chunk = 0;
 };
 
 void usage(const char* a0) {
@@ -72,6 +75,9 @@ void usage(const char* a0) {
         "  --cp-init-uniform      U[0,1) initial factors instead of N(0,1)\n"
         "  --precision P          fp32 | bf16 gathered factors (env DYN_CP_PRECISION)\n"
         "  --device D             GPU ordinal (default 0)\n"
+        "  --ooc on|off|auto      stream the tensor from host memory once per sweep\n"
+        "                         (Jacobi); auto: when the device copies do not fit\n"
+        "  --ooc-chunk NNZ        streaming chunk in nonzeros (default 2^27)\n"
         "  --save-processed P     write the processed device state (tensor + per-mode\n"
         "                         permutations) to P after the copies are built\n"
         "  --runtime              <tensor> is a processed-state file: load it and skip\n"
@@ -111,9 +117,11 @@ bool parse(int argc, char** argv, Args& a) {
         else if (k == "--precision") a.precision = need("--precision");
         else if (k == "--device") a.device = std::atoi(need("--device"));
         else if (k == "--all-modes") a.all_modes = true;
-        else if (k == "--shard-size" || k == "--layout" || k == "--ncopy-fiber-sort" ||
-                 k == "--ncopy-csr" || k == "--ncopy-csr-compact" || k == "--ooc" ||
-                 k == "--ooc-chunk") {
+        else if (k == "--ooc") a.ooc = need("--ooc");
+        else if (k == "--layout") a.layout = need("--layout");
+        else if (k == "--ooc-chunk") a.ooc_chunk = (uint64_t)std::atoll(need("--ooc-chunk"));
+        else if (k == "--shard-size" || k == "--ncopy-fiber-sort" || k == "--ncopy-csr" ||
+                 k == "--ncopy-csr-compact") {
             (void)need(k.c_str());   // CPU layout knobs: no device counterpart
         } else if (k == "--jit") {
         } else if (k == "--save-processed") {
@@ -279,19 +287,29 @@ int main(int argc, char** argv) {
         opt.device = args.device;
         opt.save_processed_path = args.save_processed;
         if (args.runtime_only) opt.processed_path = args.tensor_path;
-        ComputePlan cp = dyn_build_compute_plan(T, F.rank_padded, args.threads, "auto", "auto",
-                                                args.cp_kernel);
+        ComputePlan cp = dyn_build_compute_plan(T, F.rank_padded, args.threads, args.layout,
+                                                args.ooc, args.cp_kernel);
         if (!args.precision.empty()) {
             cp.precision = (args.precision == "bf16" || args.precision == "mixed") ? 1 : 0;
             cp.precision_source = "cli-precision";
         }
         dyn_print_compute_plan(cp, "[plan]");
-        if (!cp.fits) {
-            // the reference would stream from disk here (out-of-core Jacobi);
-            // the device engine needs the tensor and its copies resident
+        if (cp.storage.choice == Layout::OutOfCore) {
+            if (args.runtime_only) {
+                std::fprintf(stderr, "[main] --runtime and out-of-core streaming are mutually "
+                                     "exclusive\n");
+                return 7;
+            }
+            opt.stream_from_host = true;
+            opt.stream_chunk_nnz = args.ooc_chunk;
+            std::printf("[ooc] streaming the tensor from host memory in chunks of %llu "
+                        "nonzeros\n",
+                        (unsigned long long)(args.ooc_chunk ? args.ooc_chunk : (1ull << 27)));
+        } else if (!cp.fits) {
+            // --ooc off: the device engine needs the tensor and its copies resident
             std::fprintf(stderr,
                          "[plan] refusing: the device tensor and its copies need %.2f GiB, "
-                         "%.2f GiB of device memory are free\n",
+                         "%.2f GiB of device memory are free (out-of-core is off)\n",
                          cp.device_total_bytes / 1073741824.0,
                          cp.device_free_bytes / 1073741824.0);
             return 6;

@@ -88,12 +88,31 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     sp.reason = !have_dev ? "device memory unknown"
                 : p.fits  ? "per-mode sorted copies fit in device memory"
                           : "per-mode sorted copies exceed free device memory";
-    // CPU layout / out-of-core requests have no device counterpart
-    const std::string lc = lower(layout_cli), oc = lower(ooc_cli);
-    sp.override_source = (!lc.empty() && lc != "auto") ? "cli-layout (ignored: device copies)"
-                         : (!oc.empty() && oc != "auto" && oc != "off")
-                             ? "cli-ooc (ignored: device copies)"
-                             : "auto";
+    sp.override_source = "auto";
+    // out-of-core (src/dynasor_preprocess.cpp:951-1104): the tensor streams
+    // from host memory when asked (--ooc on / --layout ooc / DYN_OOC=on), or
+    // when its copies do not fit and out-of-core is not switched off.  The CPU
+    // layouts (PingPong / NCopy / InPlace / Morton) have no device counterpart.
+    const std::string lc = lower(layout_cli);
+    std::string oc = lower(ooc_cli);
+    const char* osrc = "cli-ooc";
+    if (const char* env = std::getenv("DYN_OOC"); env && env[0] && (oc.empty() || oc == "auto")) {
+        oc = lower(env);
+        osrc = "env-ooc";
+    }
+    const bool ooc_forced = oc == "on" || lc == "ooc" || lc == "outofcore" || lc == "out-of-core";
+    if (ooc_forced) {
+        sp.choice = Layout::OutOfCore;
+        sp.reason = "streamed from host memory (requested)";
+        sp.override_source = oc == "on" ? osrc : "cli-layout";
+    } else if (have_dev && !p.fits && oc != "off") {
+        sp.choice = Layout::OutOfCore;
+        sp.reason = "per-mode sorted copies exceed free device memory: streamed from host memory";
+    } else if (!lc.empty() && lc != "auto") {
+        sp.override_source = "cli-layout (ignored: device copies)";
+    }
+    sp.ooc_need = (size_t)std::min<uint64_t>(T.nnz, 1ull << 27) * (4 + 4 * (size_t)N) * 3 + dfb;
+    const bool ooc = sp.choice == Layout::OutOfCore;
 
     // ---- kernel: the reference's decision tree ---------------------------
     // (src/dynasor_preprocess.cpp:1253-1308) with the all-modes pass's
@@ -111,6 +130,10 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     }
     k = lower(k);
     auto pick_auto = [&]() -> int {
+        if (ooc) {
+            p.kernel_reason = "forced-jacobi-by-layout";
+            return 1;
+        }
         if (T.nnz < 1000000ull) {
             p.kernel_reason = "nnz<1M (linalg-dominated)";
             return 0;
@@ -133,6 +156,10 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
                k == "per_mode") {
         p.kernel_enum = 0;
         p.kernel_reason = "user-forced-gs";
+        if (ooc) {
+            p.kernel_enum = 1;
+            p.kernel_reason = "user-wanted-gs-but-layout-forces-jacobi";
+        }
     } else {
         if (!k.empty() && k != "auto")
             std::fprintf(stderr,

@@ -53,6 +53,10 @@ struct CpOptions {
     // header), and/or write the processed state after the copies are built
     std::string processed_path;
     std::string save_processed_path;
+    // out-of-HBM: stream T from host memory through device chunks of this
+    // many nonzeros (0: 2^27) once per sweep; CP-ALS then runs Jacobi
+    bool        stream_from_host = false;
+    uint64_t    stream_chunk_nnz = 0;
 };
 
 struct CpResult {

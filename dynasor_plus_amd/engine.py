@@ -111,6 +111,19 @@ class GpuContext:
                                                 _f32ptr(vals), arr), "upload_coo")
         self.N, self.dims, self.nnz, self.R = T.num_modes, tuple(T.dims), T.nnz, 0
 
+    def attach_host(self, T: Tensor, chunk_nnz: int = 0) -> None:
+        """Stream T from host memory instead of uploading it (out-of-HBM
+        tensors); T's arrays are kept referenced while attached."""
+        dims = np.array(T.dims, dtype=np.uint32)
+        cols = [np.ascontiguousarray(c, dtype=np.uint32) for c in T.idx]
+        vals = np.ascontiguousarray(T.vals, dtype=np.float32)
+        arr = (L._u32p * len(cols))(*[_u32ptr(c) for c in cols])
+        self._host_arrays = (vals, cols)
+        self._check(self.lib.dyn_gpu_attach_host_coo(self.h, T.num_modes, T.nnz, _u32ptr(dims),
+                                                     _f32ptr(vals), arr, int(chunk_nnz)),
+                    "attach_host_coo")
+        self.N, self.dims, self.nnz, self.R = T.num_modes, tuple(T.dims), T.nnz, 0
+
     def remap(self, mode: int, secondary: int = -1, want_outputs: bool = True):
         rowptr = np.zeros(self.dims[mode] + 1, dtype=np.uint64) if want_outputs else None
         perm = np.zeros(max(self.nnz, 1), dtype=np.uint32) if want_outputs else None

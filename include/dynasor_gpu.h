@@ -130,6 +130,21 @@ int dyn_gpu_remap_all(dyn_gpu_ctx* ctx);
 int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
                      uint32_t* const* idx_out);
 
+/* ---- out-of-HBM tensors (replaces the reference's out-of-core streaming,
+ *      include/dynasor_ooc.h:25-61, src/dynasor_ooc.cpp:340-535) ---------- */
+/* Attach a COO tensor that stays in caller host memory (vals[nnz],
+ * idx[n][nnz], valid until the next upload/attach or destroy); it passes
+ * through device chunk buffers of `chunk_nnz` nonzeros (0: 2^27) once per
+ * sweep.  One pass at attach checks every index (DYN_ERR_INVALID), counts the
+ * rows for the frequency relabel and sums ||T||^2.  Afterwards
+ * dyn_gpu_set_factors, dyn_gpu_mttkrp, dyn_gpu_mttkrp_all, dyn_gpu_gram and
+ * dyn_gpu_cpals (Jacobi; a Gauss-Seidel request runs Jacobi with a warning,
+ * as the reference's out-of-core path does) work; each chunk is sorted per
+ * mode on the device and its results are added in chunk order.  Entry
+ * points that need the whole tensor on the device return DYN_ERR_STATE. */
+int dyn_gpu_attach_host_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz, const uint32_t* dims,
+                            const float* vals, const uint32_t* const* idx, uint64_t chunk_nnz);
+
 /* ---- processed state (replaces save_dnp_tensor / load_dnp_tensor,
  *      include/dynasor_dnp.h:110-145) ------------------------------------ */
 /* Write the uploaded tensor and the permutation of every built copy to

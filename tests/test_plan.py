@@ -44,6 +44,7 @@ def plan_exe(tmp_path_factory):
 def plan(exe, nnz, rank, kernel, dims, have=1, free=64 * GiB, total=288 * GiB, env=None):
     e = dict(os.environ)
     e.pop("DYN_CP_KERNEL", None)
+    e.pop("DYN_OOC", None)
     e.update(env or {})
     p = subprocess.run([str(exe), str(nnz), str(rank), kernel, str(have), str(free), str(total),
                         *map(str, dims)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
@@ -82,7 +83,9 @@ def test_auto_kernel_accumulator_budget(plan_exe):
     acc = sum(dims) * 32 * 4                       # rank-ordered FP32 accumulators
     out, _ = plan(plan_exe, 2_000_000, 32, "auto", dims, free=4 * acc + 4096)
     assert int(out["kernel"]) == 1 and int(out["ofibs"]) == acc
-    out, _ = plan(plan_exe, 2_000_000, 32, "auto", dims, free=4 * acc - 4096)
+    # (out-of-core off: at this little free memory the copies would not fit either)
+    out, _ = plan(plan_exe, 2_000_000, 32, "auto", dims, free=4 * acc - 4096,
+                  env={"DYN_OOC": "off"})
     assert int(out["kernel"]) == 0 and out["reason"] == "ofibs>budget"
     # no visible device: nothing to size the pass against
     out, _ = plan(plan_exe, 2_000_000, 32, "auto", dims, have=0)
@@ -116,3 +119,25 @@ def test_fits_gate(plan_exe):
     assert int(out["fits"]) == 1 and need > 77_000_000 * 16 * 3
     out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=need - 1)
     assert int(out["fits"]) == 0
+
+
+@pytest.mark.parametrize("env,cli,kernel,reason,choice", [
+    ({"DYN_OOC": "on"}, "auto", 1, "forced-jacobi-by-layout", 3),
+    ({"DYN_OOC": "on"}, "gs", 1, "user-wanted-gs-but-layout-forces-jacobi", 3),
+    ({"DYN_OOC": "off"}, "auto", 1, "all-modes fits", 5),
+])
+def test_out_of_core_choice(plan_exe, env, cli, kernel, reason, choice):
+    """An out-of-core request streams the tensor from host memory and forces
+    Jacobi, as the reference's OutOfCore layout does."""
+    out, _ = plan(plan_exe, 2_000_000, 32, cli, (1000, 1000, 1000), env=env)
+    assert (int(out["kernel"]), out["reason"], int(out["choice"])) == (kernel, reason, choice), out
+
+
+def test_out_of_core_when_copies_do_not_fit(plan_exe):
+    dims = (12092, 9184, 28818)
+    out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims)
+    need = int(out["need"])
+    out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=need - 1)
+    assert int(out["choice"]) == 3 and int(out["kernel"]) == 1
+    out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=need - 1, env={"DYN_OOC": "off"})
+    assert int(out["choice"]) == 5 and int(out["fits"]) == 0
