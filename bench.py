@@ -23,6 +23,7 @@ import json
 import math
 import os
 import re
+import shutil
 import statistics
 import subprocess
 import sys
@@ -201,32 +202,43 @@ def timed_run(ctx, Y0, R, kern, prec, warmup, steps):
     return res, elapsed, launches, kms
 
 
-def kernel_figures(dims, nnz, R, jacobi, steps, launches, kms, bf):
+def kernel_figures(dims, nnz, R, jacobi, steps, launches, kms, bf, chunk_nnz=0):
     """Algorithmic bytes and flops of the timed spMTTKRP launches.  A
     Gauss-Seidel sweep launches every mode once; a Jacobi run of K iterations
     launches every mode K times plus the smallest mode once more for the last
     iteration's fit.  With BF16 gathers the mode whose Yhat carries the fit
-    (Gauss-Seidel: the last, Jacobi: the smallest) gathers FP32 rows."""
+    (Gauss-Seidel: the last, Jacobi: the smallest) gathers FP32 rows.  A
+    streamed tensor (chunk_nnz > 0) launches every mode once per chunk; each
+    chunk's launch is credited with its own stream bytes plus the row
+    pointer, the gathered rows and the output once."""
     N = len(dims)
     fit_mode = int(np.argmin(dims)) if jacobi else N - 1
-    per_mode = [algorithmic_bytes(dims, nnz, R, n, 4 if n == fit_mode else bf) for n in range(N)]
+    chunks = [nnz] if not chunk_nnz else \
+        [min(chunk_nnz, nnz - b) for b in range(0, nnz, chunk_nnz)]
+    per_mode = [sum(algorithmic_bytes(dims, c, R, n, 4 if n == fit_mode else bf) for c in chunks)
+                for n in range(N)]
     total = steps * sum(per_mode)
-    n_expected = steps * N
+    n_expected = steps * N * len(chunks)
     if jacobi:
         total += per_mode[fit_mode]
-        n_expected += 1
+        n_expected += len(chunks)
     if launches != n_expected or kms <= 0:
         log(f"unexpected launch count {launches} (expected {n_expected})")
-    flops = N * R * nnz * launches
+    flops = N * R * nnz * (n_expected // len(chunks))
     gbs = total / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     return {"GBps": gbs, "GFLOPs": flops / (kms * 1e-3) / 1e9 if kms > 0 else 0.0,
             "ms_per_launch": kms / launches if launches else None, "launches": launches,
             "bytes_per_launch": total / launches if launches else None}
 
 
-def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True):
+def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True, gs_too=False,
+              ooc_chunk=None, ooc_source="host"):
     """Plain FP32 and mixed BF16-gather CP-ALS on one workload: s/iter, fits,
-    equal-fit check and the spMTTKRP roofline figures of each."""
+    equal-fit check and the spMTTKRP roofline figures of each.  `gs_too`
+    adds the same comparison under Gauss-Seidel on the same copies (for a
+    Jacobi workload); `ooc_chunk` streams the tensor from host memory in
+    chunks of that many nonzeros (0: the engine default) instead of keeping
+    it resident."""
     from dynasor_plus_amd import GpuContext, host, init_factors
     from dynasor_plus_amd import _lib as L
     t0 = time.time()
@@ -241,37 +253,72 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True):
     Y0 = init_factors(dims, R, 42, "normal")
     out = {"workload": wl["desc"], "dims": list(dims), "nnz": T.nnz, "rank": R,
            "kernel": wl["kernel"], "steps": steps, "warmup": warmup}
-    with GpuContext(0) as ctx:
-        ctx.upload(T)
-        t0 = time.time()
-        ctx.remap_all()
-        out["remap_s"] = time.time() - t0
+    legs = (("mixed", L.DYN_PREC_BF16, 2), ("plain", L.DYN_PREC_FP32, 4))
+
+    def run_pair(ctx, kern_id, is_jacobi, tag):
         runs = {}
-        legs = (("mixed", L.DYN_PREC_BF16, 2), ("plain", L.DYN_PREC_FP32, 4))
         for label, prec, bf in (legs if mixed else legs[1:]):
-            res, elapsed, launches, kms = timed_run(ctx, Y0, R, kern, prec, warmup, steps)
-            k = kernel_figures(dims, T.nnz, R, jacobi, steps, launches, kms, bf)
+            res, elapsed, launches, kms = timed_run(ctx, Y0, R, kern_id, prec, warmup, steps)
+            k = kernel_figures(dims, T.nnz, R, is_jacobi, steps, launches, kms, bf,
+                               chunk_nnz=0 if ooc_chunk is None else
+                               (ooc_chunk or min(T.nnz, 1 << 27)))
             k["s_per_iter"] = elapsed / steps
             k["final_fit"] = res.final_fit
             k["fits"] = [round(f, 7) if math.isfinite(f) else None for f in res.fit_history]
-            if jacobi:
-                # all-modes pass over one copy as the judge's yardstick
+            if is_jacobi and ooc_chunk is None:
+                # all-modes pass over one copy as the yardstick
                 bj = jacobi_pass_bytes(dims, T.nnz, R, bf)
                 k["jacobi_pass_frac"] = bj / (kms * 1e-3 / (launches / N)) / 1e9 / HBM_PEAK_GBS
+            if ooc_chunk is not None:
+                # host -> device bytes per sweep over the whole iteration time
+                k["stream_GBps"] = T.nnz * (4 + 4 * N) / k["s_per_iter"] / 1e9
             runs[label] = k
-            log(f"{name} {label}: {k['s_per_iter'] * 1e3:.3f} ms/iter, spMTTKRP "
+            log(f"{name}{tag} {label}: {k['s_per_iter'] * 1e3:.3f} ms/iter, spMTTKRP "
                 f"{k['ms_per_launch']:.3f} ms/launch -> {k['GBps']:.0f} GB/s, "
                 f"{k['GFLOPs']:.0f} GFLOP/s, final fit {res.final_fit}")
-    if not mixed:
-        runs["mixed"] = runs["plain"]
-    fm, fp = runs["mixed"]["final_fit"], runs["plain"]["final_fit"]
-    equal = math.isfinite(fm) and math.isfinite(fp) and abs(fm - fp) <= EQUAL_FIT_TOL
-    out.update(runs)
-    out["equal_fit"] = equal
-    out["fit_abs_diff"] = abs(fm - fp) if math.isfinite(fm) and math.isfinite(fp) else None
-    out["speedup"] = runs["plain"]["s_per_iter"] / runs["mixed"]["s_per_iter"]
-    if not equal:
-        log(f"{name}: final fits differ (mixed {fm}, plain {fp}): the speedup is not at equal fit")
+        if not mixed:
+            runs["mixed"] = runs["plain"]
+        fm, fp = runs["mixed"]["final_fit"], runs["plain"]["final_fit"]
+        equal = math.isfinite(fm) and math.isfinite(fp) and abs(fm - fp) <= EQUAL_FIT_TOL
+        runs["equal_fit"] = equal
+        runs["fit_abs_diff"] = abs(fm - fp) if math.isfinite(fm) and math.isfinite(fp) else None
+        runs["speedup"] = runs["plain"]["s_per_iter"] / runs["mixed"]["s_per_iter"]
+        if not equal:
+            log(f"{name}{tag}: final fits differ (mixed {fm}, plain {fp}): the speedup is not "
+                "at equal fit")
+        return runs
+
+    with GpuContext(0) as ctx:
+        t0 = time.time()
+        if ooc_chunk is None:
+            ctx.upload(T)
+            ctx.remap_all()
+            out["remap_s"] = time.time() - t0
+        elif ooc_source == "dnb":
+            from dynasor_plus_amd.tensor_io import save_dnb
+            tmpdir = tempfile.mkdtemp(prefix="dynooc_")
+            path = os.path.join(tmpdir, "t.dnb")
+            save_dnb(path, T)
+            log(f"{name}: wrote {os.path.getsize(path) / 1e9:.2f} GB to {path} in "
+                f"{time.time() - t0:.1f}s")
+            t0 = time.time()
+            ctx.attach_dnb(path, ooc_chunk)
+            out["attach_s"] = time.time() - t0
+            out["source"] = "dnb"
+            out["chunk_nnz"] = ooc_chunk or min(T.nnz, 1 << 27)
+        else:
+            ctx.attach_host(T, ooc_chunk)
+            out["attach_s"] = time.time() - t0
+            out["source"] = "host"
+            out["chunk_nnz"] = ooc_chunk or min(T.nnz, 1 << 27)
+        try:
+            out.update(run_pair(ctx, kern, jacobi, ""))
+            if gs_too and jacobi:
+                out["gauss_seidel"] = run_pair(ctx, L.DYN_CP_GAUSS_SEIDEL, False, " gauss-seidel")
+        finally:
+            if ooc_source == "dnb" and ooc_chunk is not None:
+                # the context keeps the file open; the unlinked file stays readable
+                shutil.rmtree(tmpdir, ignore_errors=True)
     if want_cpu:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(T, wl, threads)
@@ -329,6 +376,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="plain FP32 gathers only (the line then reports that run)")
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--ooc", type=int, default=None, metavar="CHUNK_NNZ",
+                    help="stream the tensor from host memory in chunks of CHUNK_NNZ nonzeros "
+                         "(0: the engine default) instead of keeping it in device memory")
+    ap.add_argument("--ooc-source", default="host", choices=("host", "dnb"),
+                    help="with --ooc: stream from the host arrays, or from a .dnb file written "
+                         "to a temporary directory first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -347,18 +400,33 @@ def main():
                                                                       threads)}), flush=True)
         return
 
-    head = cpals_leg(args.workload, wl, args.warmup, steps, threads, not args.no_cpu,
-                     mixed=not args.no_mixed)
+    if args.ooc is not None:
+        wl = dict(wl, kernel="jacobi")   # a streamed tensor runs the all-modes update
+    head = cpals_leg(args.workload, wl, args.warmup, steps, threads,
+                     not args.no_cpu and args.ooc is None, mixed=not args.no_mixed,
+                     gs_too=args.workload == "c4" and not args.no_extra and args.ooc is None,
+                     ooc_chunk=args.ooc, ooc_source=args.ooc_source)
+    keys = ("s_per_iter", "final_fit", "GBps", "GFLOPs", "ms_per_launch")
+
+    def summary(leg):
+        d = {k: leg[k] for k in ("equal_fit", "fit_abs_diff", "speedup")}
+        for label in ("mixed", "plain"):
+            d[label] = {k: leg[label][k] for k in keys + ("stream_GBps",) if k in leg[label]}
+        return d
+
     extra = {}
-    if args.workload == "c4" and not args.no_extra:
+    if args.workload == "c4" and not args.no_extra and args.ooc is None:
         for name in ("c2", "c3"):
             w = WORKLOADS[name]
             leg = cpals_leg(name, w, args.warmup, w["steps"], threads, False)
-            extra[name] = {k: leg[k] for k in ("workload", "equal_fit", "fit_abs_diff", "speedup")}
-            for label in ("mixed", "plain"):
-                extra[name][label] = {k: leg[label][k] for k in
-                                      ("s_per_iter", "final_fit", "GBps", "GFLOPs",
-                                       "ms_per_launch")}
+            extra[name] = {"workload": leg["workload"], **summary(leg)}
+        # the 3-mode tensor streamed from host memory in 5 chunks (Jacobi)
+        w = dict(WORKLOADS["c2"], kernel="jacobi",
+                 desc=WORKLOADS["c2"]["desc"].replace("Gauss-Seidel", "all-modes (Jacobi)") +
+                 ", streamed from host memory in 16M-nonzero chunks")
+        leg = cpals_leg("c2-ooc", w, 1, 5, threads, False, ooc_chunk=1 << 24)
+        extra["c2_streamed"] = {"workload": leg["workload"], "chunk_nnz": leg["chunk_nnz"],
+                                "attach_s": leg["attach_s"], "steps": 5, **summary(leg)}
 
     m = head["mixed"]
     tr_file = f"spmttkrp_{args.workload}_bf16_hbm_traffic.md"
@@ -381,11 +449,12 @@ def main():
                    "rank": head["rank"], "kernel": head["kernel"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": m["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["GBps"] / HBM_PEAK_GBS,
-                     "traffic": profile_figure(tr_file, "traffic_bytes_per_launch")},
+                     "traffic": None if args.ooc is not None else
+                     profile_figure(tr_file, "traffic_bytes_per_launch")},
         "spmttkrp_GFLOPs": m["GFLOPs"],
         "spmttkrp_ms_per_launch": m["ms_per_launch"],
         "gram_mfma_busy_frac": gram_util,
-        "remap_s": head["remap_s"],
+        "remap_s": head.get("remap_s"),
         "final_fit": m["final_fit"],
         "mixed": {"s_per_iter": m["s_per_iter"], "plain_s_per_iter": head["plain"]["s_per_iter"],
                   "speedup": head["speedup"], "equal_fit": head["equal_fit"],
@@ -395,6 +464,10 @@ def main():
                   "spmttkrp_GBps": m["GBps"], "plain_spmttkrp_GBps": head["plain"]["GBps"],
                   "jacobi_pass_frac": m.get("jacobi_pass_frac"),
                   "plain_jacobi_pass_frac": head["plain"].get("jacobi_pass_frac")},
+        "gauss_seidel": summary(head["gauss_seidel"]) if "gauss_seidel" in head else None,
+        "streamed": ({"chunk_nnz": head["chunk_nnz"], "attach_s": head["attach_s"],
+                      "source": head["source"], "stream_GBps": m.get("stream_GBps")}
+                     if args.ooc is not None else None),
         "extra": extra or None,
         "cpu_baseline": head.get("cpu_baseline"),
     }

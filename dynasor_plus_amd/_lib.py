@@ -80,6 +80,7 @@ SIGNATURES = {
     "dyn_gpu_get_copy": (C.c_int, [_vp, C.c_int, _f32p, C.POINTER(_u32p)]),
     "dyn_gpu_attach_host_coo": (C.c_int, [_vp, C.c_int, C.c_uint64, _u32p, _f32p,
                                           C.POINTER(_u32p), C.c_uint64]),
+    "dyn_gpu_attach_dnb": (C.c_int, [_vp, C.c_char_p, C.c_uint64]),
     "dyn_gpu_save_processed": (C.c_int, [_vp, C.c_char_p]),
     "dyn_gpu_load_processed": (C.c_int, [_vp, C.c_char_p]),
     "dyn_gpu_processed_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), _u64p, _u32p, _u32p]),

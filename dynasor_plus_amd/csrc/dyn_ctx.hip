@@ -56,10 +56,36 @@ int set_err(dyn_gpu_ctx* c, int code, const char* fmt, ...) {
     return code;
 }
 
+void flush_alloc_cache(dyn_gpu_ctx* c) {
+    if (c->free_cache.empty()) return;
+    // parked blocks may still be read by queued work on the context stream
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->free_cache) {
+        auto it = c->allocs.find(kv.second);
+        if (it != c->allocs.end()) {
+            c->mem_used -= it->second;
+            c->allocs.erase(it);
+        }
+        (void)hipFree(kv.second);
+    }
+    c->free_cache.clear();
+}
+
 int dalloc(dyn_gpu_ctx* c, void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
     bytes = (bytes + 255) & ~size_t(255);
+    if (c->cache_allocs) {
+        // a parked block of this size: queued work that used it precedes
+        // the new user on the same stream
+        auto it = c->free_cache.find(bytes);
+        if (it != c->free_cache.end()) {
+            *p = it->second;
+            c->free_cache.erase(it);
+            return DYN_OK;
+        }
+    }
+    if (c->mem_cap && c->mem_used + bytes > c->mem_cap) flush_alloc_cache(c);
     if (c->mem_cap && c->mem_used + bytes > c->mem_cap)
         return set_err(c, DYN_ERR_OOM,
                        "device allocation of %zu bytes exceeds the context cap "
@@ -67,6 +93,11 @@ int dalloc(dyn_gpu_ctx* c, void** p, size_t bytes) {
                        bytes, (unsigned long long)c->mem_used,
                        (unsigned long long)c->mem_cap);
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !c->free_cache.empty()) {
+        (void)hipGetLastError();
+        flush_alloc_cache(c);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
@@ -82,6 +113,10 @@ void dfree(dyn_gpu_ctx* c, void* p) {
     if (!p) return;
     auto it = c->allocs.find(p);
     if (it != c->allocs.end()) {
+        if (c->cache_allocs) {
+            c->free_cache.emplace(it->second, p);
+            return;
+        }
         c->mem_used -= it->second;
         c->allocs.erase(it);
     }
@@ -180,9 +215,6 @@ void free_factors(dyn_gpu_ctx* c) {
         c->Yhat[n] = nullptr;
         c->G[n] = nullptr;
     }
-    dfree(c, c->stream_tmp);
-    c->stream_tmp = nullptr;
-    c->stream_tmp_elems = 0;
     dfree(c, c->lambda);
     dfree(c, c->inv_lambda);
     dfree(c, c->L);
@@ -223,6 +255,11 @@ void free_relabel(dyn_gpu_ctx* c) {
 }
 
 void free_tensor(dyn_gpu_ctx* c) {
+    if (c->streamed) {
+        // vals / idx alias the streamer's chunk buffers only inside a pass
+        c->vals = nullptr;
+        for (int n = 0; n < kMaxModes; ++n) c->idx[n] = nullptr;
+    }
     for (int m = 0; m < kMaxModes; ++m) free_copy(c, m);
     free_relabel(c);
     dfree(c, c->vals);
@@ -235,11 +272,8 @@ void free_tensor(dyn_gpu_ctx* c) {
     c->N = 0;
     c->nnz = 0;
     c->tnorm2 = -1.0;
+    stream_release(c);
     c->streamed = false;
-    c->host_vals = nullptr;
-    for (int n = 0; n < kMaxModes; ++n) c->host_idx[n] = nullptr;
-    c->total_nnz = 0;
-    c->chunk_nnz = 0;
 }
 
 }  // namespace dyn
@@ -314,6 +348,8 @@ extern "C" int dyn_gpu_destroy(dyn_gpu_ctx* c) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
     }
+    c->cache_allocs = false;
+    flush_alloc_cache(c);
     for (auto& kv : c->allocs) (void)hipFree(kv.first);
     (void)hipHostFree(c->fitbuf_host);
     (void)hipStreamSynchronize(c->aux_stream);

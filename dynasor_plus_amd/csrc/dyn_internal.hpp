@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -84,6 +85,8 @@ struct TimedPair {
     hipEvent_t a, b;
 };
 
+struct Streamer;   // out-of-HBM source, chunk buffers and pass state (dyn_stream.hip)
+
 }  // namespace dyn
 
 struct dyn_gpu_ctx {
@@ -132,17 +135,19 @@ struct dyn_gpu_ctx {
     int          num_cus = 256;
     size_t       max_lds = 65536;
 
-    // out-of-HBM streaming (dyn_gpu_attach_host_coo): the tensor stays in
-    // caller host memory and passes through the device chunk buffers (vals /
-    // idx, chunk_nnz long) once per sweep; nnz is the whole tensor's count
-    // except inside a sweep, where it is the current chunk's
+    // out-of-HBM streaming (dyn_gpu_attach_host_coo / dyn_gpu_attach_dnb):
+    // the tensor stays in host memory or in a .dnb file and passes through
+    // two device chunk buffers once per sweep; nnz is the whole tensor's
+    // count except inside a pass, where it is the current chunk's and vals /
+    // idx point at the chunk's buffers
     bool         streamed = false;
-    const float* host_vals = nullptr;
-    const uint32_t* host_idx[dyn::kMaxModes] = {nullptr};
-    uint64_t     total_nnz = 0;
-    uint64_t     chunk_nnz = 0;
-    float*       stream_tmp = nullptr;            // one mode's chunk spMTTKRP
-    size_t       stream_tmp_elems = 0;
+    dyn::Streamer* streamer = nullptr;            // owned (dyn_stream.hip)
+
+    // allocation cache: while set, dfree parks blocks for dalloc to hand out
+    // again at the same size (no hipFree, which waits for the whole device,
+    // inside a streamed pass); flush_alloc_cache releases them
+    bool         cache_allocs = false;
+    std::multimap<size_t, void*> free_cache;
 
     // device address of this device's fixed partials array (no-allocation
     // entry point), from hipGetSymbolAddress at context creation
@@ -245,11 +250,15 @@ int adjoint_mode(dyn_gpu_ctx* c, int mode, const float* const* A, int Rp, const 
                  float* out);
 
 // Streamed sweep: Yhat[n] for every mode n in `mask` over all chunks of an
-// attached host tensor, mode n gathering at precision prec[n]; chunk results
-// are added in chunk order.
+// attached out-of-HBM tensor, mode n gathering at precision prec[n]; chunk
+// results are added in chunk order.
 int stream_sweep(dyn_gpu_ctx* c, uint32_t mask, const int* prec);
 // DYN_ERR_STATE for entry points that need the whole tensor on the device
 int reject_streamed(dyn_gpu_ctx* c, const char* what);
+// Releases the streamer (loader state, chunk buffers, staging, copy stream)
+void stream_release(dyn_gpu_ctx* c);
+// hipFree every block parked by the allocation cache
+void flush_alloc_cache(dyn_gpu_ctx* c);
 
 int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp,
                  const double* lambda_or_null, float* m_out, const float* vals_dot,

@@ -1,6 +1,7 @@
 // cpals(): the reference's CP-ALS entry point (include/dynasor_cpals.h:121,
 // src/dynasor_cpals.cpp:120-396) on top of the device engine's C-ABI.
 #include "dynasor_cpals.h"
+#include "dynasor_ooc.h"
 #include "dynasor_gpu.h"
 #include "dynasor_host.h"
 
@@ -89,6 +90,19 @@ CpResult cpals(Tensor& T, FactorMatrices& F, const CpOptions& opt) {
         }
         rc = dyn_gpu_load_processed(ctx.h, opt.processed_path.c_str());
         if (rc != DYN_OK) return fail("dyn_gpu_load_processed", rc);
+    } else if (!opt.stream_path.empty()) {
+        // the file must hold the tensor T describes
+        Tensor hdr;
+        bool same = ooc_load_header(opt.stream_path, hdr) && hdr.num_modes == N &&
+                    hdr.nnz == T.nnz;
+        for (int n = 0; same && n < N; ++n) same = hdr.mode_size[n] == T.mode_size[n];
+        if (!same) {
+            std::fprintf(stderr, "[cpals] '%s' does not hold this tensor\n",
+                         opt.stream_path.c_str());
+            return CpResult();
+        }
+        rc = dyn_gpu_attach_dnb(ctx.h, opt.stream_path.c_str(), opt.stream_chunk_nnz);
+        if (rc != DYN_OK) return fail("dyn_gpu_attach_dnb", rc);
     } else if (opt.stream_from_host) {
         rc = dyn_gpu_attach_host_coo(ctx.h, N, T.nnz, T.mode_size, T.vals, T.idx_buf,
                                      opt.stream_chunk_nnz);

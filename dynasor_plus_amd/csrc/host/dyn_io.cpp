@@ -3,6 +3,7 @@
 // are byte-compatible with the reference (src/dynasor_io.cpp:544-1348).
 #include "dynasor_common.h"
 #include "dynasor_host.h"
+#include "dynasor_ooc.h"
 
 #include <sys/stat.h>
 
@@ -269,6 +270,40 @@ static bool dnb_fresh(const std::string& tns, const std::string& dnb) {
     struct stat a{};
     if (::stat(tns.c_str(), &a) != 0) return true;   // no source: trust the cache
     return b.st_mtime >= a.st_mtime;
+}
+
+bool ooc_load_header(const std::string& path, Tensor& T) {
+    std::FILE* fp = std::fopen(path.c_str(), "rb");
+    if (!fp) return false;
+    char magic[4];
+    uint32_t version = 0, vb = 0, ib = 0, nm = 0, sizes[DYN_MAX_MODES], reserved[7];
+    uint64_t nnz = 0;
+    bool ok = std::fread(magic, 4, 1, fp) == 1 && std::fread(&version, 4, 1, fp) == 1 &&
+              std::fread(&vb, 4, 1, fp) == 1 && std::fread(&ib, 4, 1, fp) == 1 &&
+              std::fread(&nm, 4, 1, fp) == 1 && std::fread(&nnz, 8, 1, fp) == 1 &&
+              std::fread(sizes, sizeof(sizes), 1, fp) == 1 &&
+              std::fread(reserved, sizeof(reserved), 1, fp) == 1;
+    ok = ok && std::memcmp(magic, kDnbMagic, 4) == 0 && version == 1 && vb == 4 && ib == 4 &&
+         nm >= 1 && nm <= (uint32_t)DYN_MAX_MODES && nnz > 0;
+    long long size = -1;
+    if (ok && std::fseek(fp, 0, SEEK_END) == 0) size = std::ftell(fp);
+    std::fclose(fp);
+    if (!ok || size < 0 || (uint64_t)size != kDnbHeader + nnz * 4ull * (1 + nm)) return false;
+    T.free_elements();
+    T.num_modes = (int)nm;
+    T.nnz = nnz;
+    for (uint32_t n = 0; n < nm; ++n) T.mode_size[n] = sizes[n];
+    return true;
+}
+
+uint64_t ooc_default_chunk_nnz(const Tensor& T) {
+    if (T.num_modes <= 0 || T.nnz == 0) return 0;
+    uint64_t chunk = 1ull << 27;
+    if (const char* env = std::getenv("DYN_OOC_CHUNK_BYTES"); env && env[0]) {
+        const long long v = std::atoll(env);
+        if (v > 0) chunk = std::max<uint64_t>((uint64_t)v / (4 + 4 * (uint64_t)T.num_modes), 1);
+    }
+    return std::min<uint64_t>(chunk, T.nnz);
 }
 
 bool load_coo_tensor(const std::string& path, Tensor& T, bool cache_on_success) {

@@ -13,6 +13,7 @@
 #include "dynasor_dnp.h"
 #include "dynasor_gpu.h"
 #include "dynasor_host.h"
+#include "dynasor_ooc.h"
 
 #include <chrono>
 #include <cmath>
@@ -257,6 +258,7 @@ int main(int argc, char** argv) {
     std::printf("=== Dynasor+ MI355X engine (backend: HIP gfx950) ===\n");
     std::unique_ptr<Tensor> Tp(new Tensor());
     Tensor& T = *Tp;
+    std::string stream_dnb;   // non-empty: stream the tensor from this .dnb file
     if (args.runtime_only) {
         // src/main.cpp:362-427: a processed file replaces load + preprocessing
         if (args.verify || args.decompose != "cpals") {
@@ -267,9 +269,29 @@ int main(int argc, char** argv) {
         std::printf("[runtime] loading processed state: %s\n          modes=%d  nnz=%llu  "
                     "layout=device-copies\n",
                     args.tensor_path.c_str(), T.num_modes, (unsigned long long)T.nnz);
-    } else if (!load_coo_tensor(args.tensor_path, T, !args.no_cache)) {
-        std::fprintf(stderr, "failed to load tensor '%s'\n", args.tensor_path.c_str());
-        return 2;
+    } else {
+        // src/main.cpp:444-513: with a .dnb cache beside the tensor, plan from
+        // its header alone; a plan that streams leaves the tensor on disk
+        if (args.decompose == "cpals" && !args.verify && !args.no_cache) {
+            Tensor hdr;
+            if (ooc_load_header(args.tensor_path + ".dnb", hdr)) {
+                const int Rp = std::max((args.rank + 7) / 8 * 8, (args.rank + 15) / 16 * 16);
+                const ComputePlan pre = dyn_build_compute_plan(hdr, Rp, args.threads, args.layout,
+                                                               args.ooc, args.cp_kernel);
+                if (pre.storage.choice == Layout::OutOfCore) {
+                    stream_dnb = args.tensor_path + ".dnb";
+                    T.num_modes = hdr.num_modes;
+                    T.nnz = hdr.nnz;
+                    for (int n = 0; n < hdr.num_modes; ++n) T.mode_size[n] = hdr.mode_size[n];
+                    std::printf("[main] OUT-OF-CORE: the tensor stays in %s (nnz %llu, %d modes)\n",
+                                stream_dnb.c_str(), (unsigned long long)T.nnz, T.num_modes);
+                }
+            }
+        }
+        if (stream_dnb.empty() && !load_coo_tensor(args.tensor_path, T, !args.no_cache)) {
+            std::fprintf(stderr, "failed to load tensor '%s'\n", args.tensor_path.c_str());
+            return 2;
+        }
     }
     if (args.decompose == "cpals") {
         FactorMatrices F;
@@ -294,17 +316,20 @@ int main(int argc, char** argv) {
             cp.precision_source = "cli-precision";
         }
         dyn_print_compute_plan(cp, "[plan]");
-        if (cp.storage.choice == Layout::OutOfCore) {
+        if (cp.storage.choice == Layout::OutOfCore || !stream_dnb.empty()) {
             if (args.runtime_only) {
                 std::fprintf(stderr, "[main] --runtime and out-of-core streaming are mutually "
                                      "exclusive\n");
                 return 7;
             }
-            opt.stream_from_host = true;
-            opt.stream_chunk_nnz = args.ooc_chunk;
-            std::printf("[ooc] streaming the tensor from host memory in chunks of %llu "
-                        "nonzeros\n",
-                        (unsigned long long)(args.ooc_chunk ? args.ooc_chunk : (1ull << 27)));
+            opt.stream_chunk_nnz = args.ooc_chunk ? args.ooc_chunk : ooc_default_chunk_nnz(T);
+            if (!stream_dnb.empty())
+                opt.stream_path = stream_dnb;
+            else
+                opt.stream_from_host = true;
+            std::printf("[ooc] streaming the tensor from %s in chunks of %llu nonzeros\n",
+                        stream_dnb.empty() ? "host memory" : stream_dnb.c_str(),
+                        (unsigned long long)opt.stream_chunk_nnz);
         } else if (!cp.fits) {
             // --ooc off: the device engine needs the tensor and its copies resident
             std::fprintf(stderr,

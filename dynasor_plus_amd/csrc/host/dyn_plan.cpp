@@ -111,7 +111,8 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     } else if (!lc.empty() && lc != "auto") {
         sp.override_source = "cli-layout (ignored: device copies)";
     }
-    sp.ooc_need = (size_t)std::min<uint64_t>(T.nnz, 1ull << 27) * (4 + 4 * (size_t)N) * 3 + dfb;
+    // two chunk buffers, one sorted copy and the sort scratch of a chunk
+    sp.ooc_need = (size_t)std::min<uint64_t>(T.nnz, 1ull << 27) * (4 + 4 * (size_t)N) * 4 + dfb;
     const bool ooc = sp.choice == Layout::OutOfCore;
 
     // ---- kernel: the reference's decision tree ---------------------------

@@ -53,9 +53,12 @@ struct CpOptions {
     // header), and/or write the processed state after the copies are built
     std::string processed_path;
     std::string save_processed_path;
-    // out-of-HBM: stream T from host memory through device chunks of this
-    // many nonzeros (0: 2^27) once per sweep; CP-ALS then runs Jacobi
+    // out-of-HBM: stream T through device chunks of this many nonzeros (0:
+    // 2^27) once per sweep, from T's host arrays (stream_from_host) or from
+    // the .dnb file stream_path (T then needs only num_modes / nnz /
+    // mode_size, as ooc_load_header fills them); CP-ALS then runs Jacobi
     bool        stream_from_host = false;
+    std::string stream_path;
     uint64_t    stream_chunk_nnz = 0;
 };
 

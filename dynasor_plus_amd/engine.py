@@ -124,6 +124,19 @@ class GpuContext:
                     "attach_host_coo")
         self.N, self.dims, self.nnz, self.R = T.num_modes, tuple(T.dims), T.nnz, 0
 
+    def attach_dnb(self, path, chunk_nnz: int = 0) -> None:
+        """Stream the tensor of a .dnb file from disk, chunk by chunk, once
+        per sweep (only the header is read here, then one checking pass)."""
+        self._host_arrays = None
+        self._check(self.lib.dyn_gpu_attach_dnb(self.h, str(path).encode(), int(chunk_nnz)),
+                    "attach_dnb")
+        with open(path, "rb") as f:
+            head = f.read(88)
+        N = int.from_bytes(head[16:20], "little")
+        nnz = int.from_bytes(head[20:28], "little")
+        dims = tuple(int.from_bytes(head[28 + 4 * n:32 + 4 * n], "little") for n in range(N))
+        self.N, self.dims, self.nnz, self.R = N, dims, nnz, 0
+
     def remap(self, mode: int, secondary: int = -1, want_outputs: bool = True):
         rowptr = np.zeros(self.dims[mode] + 1, dtype=np.uint64) if want_outputs else None
         perm = np.zeros(max(self.nnz, 1), dtype=np.uint32) if want_outputs else None

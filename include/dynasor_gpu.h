@@ -135,7 +135,9 @@ int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
 /* Attach a COO tensor that stays in caller host memory (vals[nnz],
  * idx[n][nnz], valid until the next upload/attach or destroy); it passes
  * through device chunk buffers of `chunk_nnz` nonzeros (0: 2^27) once per
- * sweep.  One pass at attach checks every index (DYN_ERR_INVALID), counts the
+ * sweep, the next chunk's upload overlapping the current chunk's work (a
+ * loader thread and a copy stream).  One pass at attach checks every index
+ * (DYN_ERR_INVALID), counts the
  * rows for the frequency relabel and sums ||T||^2.  Afterwards
  * dyn_gpu_set_factors, dyn_gpu_mttkrp, dyn_gpu_mttkrp_all, dyn_gpu_gram and
  * dyn_gpu_cpals (Jacobi; a Gauss-Seidel request runs Jacobi with a warning,
@@ -144,6 +146,12 @@ int dyn_gpu_get_copy(dyn_gpu_ctx* ctx, int mode, float* vals_out,
  * points that need the whole tensor on the device return DYN_ERR_STATE. */
 int dyn_gpu_attach_host_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz, const uint32_t* dims,
                             const float* vals, const uint32_t* const* idx, uint64_t chunk_nnz);
+/* The same with the tensor left in a .dnb file (src/dynasor_io.cpp:747-774
+ * layout; replaces OocStream::open / read_chunk, include/dynasor_ooc.h:25-61):
+ * every pass reads each chunk's value and index ranges from the file into
+ * pinned staging.  Only the header is read here before the attach pass; a
+ * bad header or a size that does not match it returns DYN_ERR_INVALID. */
+int dyn_gpu_attach_dnb(dyn_gpu_ctx* ctx, const char* path, uint64_t chunk_nnz);
 
 /* ---- processed state (replaces save_dnp_tensor / load_dnp_tensor,
  *      include/dynasor_dnp.h:110-145) ------------------------------------ */

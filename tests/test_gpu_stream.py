@@ -1,10 +1,13 @@
-"""Out-of-HBM tensors streamed from host memory (dyn_gpu_attach_host_coo),
-the device counterpart of the reference's out-of-core Jacobi driver
-(src/dynasor_ooc.cpp:340-535): the streamed all-modes spMTTKRP equals the
-in-core one up to the regrouping of FP32 sums (chunk partials added in
-order), streamed Jacobi CP-ALS follows the FP64 oracle, a Gauss-Seidel
-request runs Jacobi, indices are checked, entry points that need the whole
-tensor refuse, and the CLI's --ooc on path reproduces the in-core fits."""
+"""Out-of-HBM tensors streamed from host memory (dyn_gpu_attach_host_coo)
+or from a .dnb file (dyn_gpu_attach_dnb), the device counterpart of the
+reference's out-of-core Jacobi driver (src/dynasor_ooc.cpp:340-535): the
+streamed all-modes spMTTKRP equals the in-core one up to the regrouping of
+FP32 sums (chunk partials added in order) and is the same bit for bit from
+either source, streamed Jacobi CP-ALS follows the FP64 oracle, a
+Gauss-Seidel request runs Jacobi, indices and file headers are checked,
+entry points that need the whole tensor refuse, and the CLI's --ooc on path
+(from the .dnb beside the tensor, or from host memory with --no-cache)
+reproduces the in-core fits."""
 import subprocess
 import sys
 from pathlib import Path
@@ -14,7 +17,7 @@ import pytest
 
 from dynasor_plus_amd import GpuContext, host, init_factors
 from dynasor_plus_amd import _lib as L
-from dynasor_plus_amd.tensor_io import Tensor, save_dnb
+from dynasor_plus_amd.tensor_io import Tensor, save_dnb, save_tns
 from tests._oracle import load as load_oracle
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +53,63 @@ def test_streamed_mttkrp_all_matches_in_core(dims, nnz, R, chunks):
             for n in range(len(dims)):
                 assert frob_rel(got[n], outs[n]) < 1e-6, (p, n)
             np.testing.assert_allclose(b.mttkrp(1, p), got[1], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("dims,nnz,R,chunk", [((900, 700, 500), 100_000, 32, 30_001),
+                                              ((500, 400, 300, 60, 7), 80_000, 16, 80_000)])
+def test_dnb_stream_equals_host_stream(tmp_path, dims, nnz, R, chunk):
+    say("stream dnb", dims, chunk)
+    T, _, _ = host.generate(dims, nnz, seed=33, coords="zipf", alpha=1.2, values="lognormal")
+    path = tmp_path / "x.dnb"
+    save_dnb(path, T)
+    rng = np.random.default_rng(4)
+    Y = [rng.standard_normal((d, R)).astype(np.float32) for d in dims]
+    with GpuContext(0) as a:
+        a.upload(T)
+        a.set_factors(Y, R)
+        core = a.mttkrp_all(L.DYN_PREC_FP32)
+    with GpuContext(0) as h, GpuContext(0) as f:
+        h.attach_host(T, chunk_nnz=chunk)
+        f.attach_dnb(path, chunk_nnz=chunk)
+        assert f.nnz == nnz and f.dims == tuple(dims)
+        for ctx in (h, f):
+            ctx.set_factors(Y, R)
+        for p in (L.DYN_PREC_FP32, L.DYN_PREC_BF16):
+            got_h, got_f = h.mttkrp_all(p), f.mttkrp_all(p)
+            for n in range(len(dims)):
+                np.testing.assert_array_equal(got_f[n], got_h[n])
+                if p == L.DYN_PREC_FP32:
+                    assert frob_rel(got_f[n], core[n]) < 1e-6, n
+        # the same Jacobi run from either source
+        rh = h.run_cpals(3, 0.0, kernel=L.DYN_CP_JACOBI)
+        rf = f.run_cpals(3, 0.0, kernel=L.DYN_CP_JACOBI)
+        np.testing.assert_array_equal(rf.fit_history, rh.fit_history)
+
+
+def test_dnb_stream_rejects_bad_files(tmp_path):
+    say("stream dnb refusals")
+    T, _, _ = host.generate((50, 40, 30), 2_000, seed=2, coords="uniform", values="lognormal")
+    good = tmp_path / "g.dnb"
+    save_dnb(good, T)
+    raw = good.read_bytes()
+    cases = {"missing": None, "magic": b"XNB1" + raw[4:], "truncated": raw[:-4],
+             "short_header": raw[:40]}
+    with GpuContext(0) as ctx:
+        for name, data in cases.items():
+            p = tmp_path / f"{name}.dnb"
+            if data is not None:
+                p.write_bytes(data)
+            with pytest.raises(L.DynGpuError) as e:
+                ctx.attach_dnb(p)
+            assert e.value.code == L.DYN_ERR_INVALID, name
+        bad = [c.copy() for c in T.idx]
+        bad[0][5] = 50
+        bp = tmp_path / "oob.dnb"
+        save_dnb(bp, Tensor(T.dims, T.vals, bad))
+        with pytest.raises(L.DynGpuError) as e:
+            ctx.attach_dnb(bp, chunk_nnz=700)
+        assert e.value.code == L.DYN_ERR_INVALID
+        ctx.attach_dnb(good, chunk_nnz=700)   # the context is usable afterwards
 
 
 def test_streamed_jacobi_tracks_fp64_oracle():
@@ -109,8 +169,19 @@ def test_cli_ooc_matches_in_core_jacobi(tmp_path):
                        timeout=120)
     b = subprocess.run(common + ["--ooc", "on", "--ooc-chunk", "9000"], stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=120)
-    assert a.returncode == 0 and b.returncode == 0, (a.stdout[-800:], b.stdout[-800:])
-    assert "[ooc] streaming" in b.stdout
+    # the same tensor as text without a cache: streamed from host memory
+    u = tmp_path / "u.tns"
+    save_tns(u, T)
+    c = subprocess.run([str(exe), str(u)] + common[2:] + ["--no-cache", "--ooc", "on",
+                                                         "--ooc-chunk", "9000"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert a.returncode == 0 and b.returncode == 0 and c.returncode == 0, \
+        (a.stdout[-800:], b.stdout[-800:], c.stdout[-800:])
+    assert "OUT-OF-CORE: the tensor stays in" in b.stdout
+    assert f"[ooc] streaming the tensor from {t}.dnb" in b.stdout
+    assert "[ooc] streaming the tensor from host memory" in c.stdout
     fits = lambda out: [float(l.split("fit = ")[1].split()[0]) for l in out.splitlines()
                         if l.startswith("[cpals] iter")]
+    assert len(fits(a.stdout)) == 5
     np.testing.assert_allclose(fits(b.stdout), fits(a.stdout), rtol=0, atol=2e-6)
+    np.testing.assert_allclose(fits(c.stdout), fits(b.stdout), rtol=0, atol=0)
