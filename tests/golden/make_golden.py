@@ -5,7 +5,8 @@ Runs oracle/_ref/dynasor_plus (compiled from the reference sources by
 files next to this script.  Only needed when the fixtures change; the tests
 read the .npz files and never run the reference.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        # every fixture
+    python tests/golden/make_golden.py c2     # only the C2 fit record
 
 Fixtures
   verify_<name>_r<R>.npz   --verify runs: tensor, R, seed, Yhat per mode
@@ -106,6 +107,14 @@ def c2_fits(tmp):
     rec = {"dims": [12092, 9184, 28818], "nnz": 77_000_000, "seed": 2, "alpha": 1.1,
            "values": "lognormal", "rank": 32, "init_seed": 42, "checksum": checksum(T),
            "fits": fits, "threads": "reference default (all host threads)"}
+    # the reference's own fits at several thread counts: its FP32 Gram, lambda
+    # and fit reductions follow the OpenMP thread order
+    rec["fits_by_threads"] = {}
+    for th in (1, 4, 8):
+        out = run([p, "--decompose", "cpals", "--cp-kernel", "gauss-seidel", "--rank", 32,
+                   "--cp-iters", 3, "--cp-tol", 0, "--seed", 42, "--threads", th], tmp)
+        rec["fits_by_threads"][str(th)] = [float(m.group(2)) for m in
+                                           (FIT_RE.match(l) for l in out.splitlines()) if m]
     # the oracle's trajectories from the same init: FP64 algebra (what the
     # device computes) and the reference's FP32 algebra
     sys.path.insert(0, str(ROOT))
@@ -123,6 +132,10 @@ def c2_fits(tmp):
 def main():
     if not REF_BIN.exists():
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["c2"]:
+        with tempfile.TemporaryDirectory() as tmp:
+            c2_fits(tmp)
+        return
     with tempfile.TemporaryDirectory() as tmp:
         d37 = REF_DATA / "3d_7.tns"
         scp = REF_DATA / "synth_cp_r5.tns"
