@@ -24,6 +24,8 @@ DYN_PREC_FP32 = 0
 DYN_PREC_BF16 = 1
 DYN_CP_GAUSS_SEIDEL = 0
 DYN_CP_JACOBI = 1
+DYN_LAYOUT_COPIES = 0
+DYN_LAYOUT_LINEAR = 1
 DYN_LOSS_LS = 0
 DYN_LOSS_BERNOULLI = 1
 MAX_MODES = 8
@@ -91,6 +93,10 @@ SIGNATURES = {
     "dyn_gpu_mttkrp_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int]),
     "dyn_gpu_mttkrp_all": (C.c_int, [_vp, C.c_int, C.POINTER(_f32p), C.c_int]),
     "dyn_gpu_gram": (C.c_int, [_vp, C.c_int, _f64p]),
+    "dyn_linear_plan": (C.c_int, [C.c_int, _u32p, _u64p, C.POINTER(C.c_uint8),
+                                  C.POINTER(C.c_uint8)]),
+    "dyn_gpu_build_linear": (C.c_int, [_vp, _u32p]),
+    "dyn_gpu_set_layout": (C.c_int, [_vp, C.c_int]),
     "dyn_gpu_cpals": (C.c_int, [_vp, C.POINTER(CpOpts), C.POINTER(CpResultC)]),
     "dyn_gpu_forward": (C.c_int, [_vp, C.c_int, _f32p]),
     "dyn_gpu_adjoint": (C.c_int, [_vp, _f32p, C.c_int, _f32p, C.c_int]),

@@ -784,7 +784,6 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     };
     const int N = c->N;
     bool jacobi = o->kernel == DYN_CP_JACOBI;
-    const bool bf = o->precision == DYN_PREC_BF16;
     if (c->streamed && !jacobi) {
         // src/dynasor_cpals.cpp:223-231: the streaming driver is all-modes only
         std::fprintf(stderr,
@@ -792,6 +791,16 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                      "no streaming driver: running Jacobi (all-modes) instead\n");
         jacobi = true;
     }
+    if (c->use_linear && !jacobi) {
+        // src/dynasor_cpals.cpp:232-240: the linearised layout is all-modes only
+        std::fprintf(stderr,
+                     "[cpals] WARN: the tensor is linearised (Z-order); Gauss-Seidel needs the "
+                     "per-mode copies: running Jacobi (all-modes) instead\n");
+        jacobi = true;
+    }
+    // the linearised pass gathers FP32 rows (it is bound by its scattered FP32
+    // additions, so BF16 operands would not shorten it)
+    const bool bf = o->precision == DYN_PREC_BF16 && !c->use_linear;
 
     double* hist = res->fit_history;
     double* itim = res->iter_time_s;
@@ -800,8 +809,12 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     res->iter_time_s = itim;
 
     auto t_r0 = clk::now();
-    for (int m = 0; m < N && !c->streamed; ++m)
-        if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
+    if (c->use_linear) {
+        if (!c->lin.built) DYN_CHECK(build_linear(c));
+    } else {
+        for (int m = 0; m < N && !c->streamed; ++m)
+            if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));
+    }
     DYN_HIP(c, hipStreamSynchronize(c->stream));
     res->remap_time_s = secs(t_r0, clk::now());
 
@@ -910,14 +923,15 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
     const int fit_mode = jacobi ? s_mode : N - 1;
     auto mttkrp_n = [&](int n) {
         const bool f32 = bf && n == fit_mode;
-        return mttkrp_mode(c, n, f32 ? DYN_PREC_FP32 : o->precision, f32 ? Yin32 : Yin, nullptr,
-                           c->Yhat[n], c->Rp);
+        const int prec = (f32 || !bf) ? DYN_PREC_FP32 : DYN_PREC_BF16;
+        return mttkrp_mode(c, n, prec, f32 ? Yin32 : Yin, nullptr, c->Yhat[n], c->Rp);
     };
     int precs[kMaxModes];
     for (int m = 0; m < N; ++m) precs[m] = (bf && m != fit_mode) ? DYN_PREC_BF16 : DYN_PREC_FP32;
     // the spMTTKRPs of the modes in `mask` (Jacobi), in core or streamed
     auto sweep = [&](uint32_t mask) -> int {
         if (c->streamed) return stream_sweep(c, mask, precs);
+        if (c->use_linear) return linear_sweep(c, mask);
         for (int m = 0; m < N; ++m)
             if ((mask >> m) & 1u) DYN_CHECK(mttkrp_n(m));
         return DYN_OK;

@@ -249,6 +249,7 @@ void free_relabel(dyn_gpu_ctx* c) {
         dfree(c, c->relab[n]);
         c->relab[n] = nullptr;
         c->relab_inv_host[n].clear();
+        c->rank_count_host[n].clear();
     }
     c->relabeled = false;
     c->gather_valid = false;
@@ -261,6 +262,8 @@ void free_tensor(dyn_gpu_ctx* c) {
         for (int n = 0; n < kMaxModes; ++n) c->idx[n] = nullptr;
     }
     for (int m = 0; m < kMaxModes; ++m) free_copy(c, m);
+    free_linear(c);
+    c->use_linear = false;
     free_relabel(c);
     dfree(c, c->vals);
     c->vals = nullptr;
@@ -663,6 +666,8 @@ extern "C" int dyn_gpu_mttkrp(dyn_gpu_ctx* c, int mode, int prec, float* out, in
         int precs[kMaxModes];
         for (int m = 0; m < c->N; ++m) precs[m] = prec;
         DYN_CHECK(stream_sweep(c, 1u << mode, precs));
+    } else if (c->use_linear) {
+        DYN_CHECK(linear_sweep(c, 1u << mode));
     } else {
         if (!c->copy[mode].built) DYN_CHECK(remap_mode(c, mode, mode == 0 ? 1 : 0));
         const void* Yin[kMaxModes];
@@ -716,6 +721,8 @@ extern "C" int dyn_gpu_mttkrp_all(dyn_gpu_ctx* c, int prec, float* const* out, i
         int precs[kMaxModes];
         for (int m = 0; m < c->N; ++m) precs[m] = prec;
         DYN_CHECK(stream_sweep(c, (1u << c->N) - 1u, precs));
+    } else if (c->use_linear) {
+        DYN_CHECK(linear_sweep(c, (1u << c->N) - 1u));
     } else {
         for (int m = 0; m < c->N; ++m)
             if (!c->copy[m].built) DYN_CHECK(remap_mode(c, m, m == 0 ? 1 : 0));

@@ -75,6 +75,24 @@ struct ModeCopy {
     uint32_t* empty_rows = nullptr;             // nempty
 };
 
+// The linearised (Z-order) copy: every nonzero once, ordered by a 64-bit key
+// that interleaves the bits of the modes' frequency ranks, with explicit rank
+// columns for every mode.  One pass over it produces every mode's spMTTKRP
+// (dyn_linear.hip).  The key only orders the copy; coordinates stay exact
+// even when the ranks have more than 64 bits in total (the key then uses the
+// leading bits of the largest modes' ranks).
+struct LinearCopy {
+    bool      built = false;
+    float*    vals = nullptr;                   // nnz, Z-order
+    uint32_t* idx[kMaxModes] = {nullptr};      // relab[n][idx[n]], Z-order
+    uint32_t* perm = nullptr;                   // Z-order position -> input position
+    uint64_t  mask[kMaxModes] = {0};           // key bits of mode n
+    uint8_t   bits[kMaxModes] = {0};           // bits of mode n's rank (>= 1)
+    uint8_t   shift[kMaxModes] = {0};          // low rank bits left out of the key
+    float*    acc = nullptr;                    // rank-order accumulators, sum_n I_n x Rp
+    size_t    acc_floats = 0;
+};
+
 constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a piece
 constexpr uint32_t kPiece = 64 * kPieceLane;         // nonzeros per heavy piece (one wave)
 constexpr uint32_t kHeavyMin = 256;                  // longer rows are heavy
@@ -130,6 +148,12 @@ struct dyn_gpu_ctx {
     bool         relabeled = false;
     uint32_t*    relab[dyn::kMaxModes] = {nullptr};
     std::vector<uint32_t> relab_inv_host[dyn::kMaxModes];
+    std::vector<uint32_t> rank_count_host[dyn::kMaxModes];   // nonzeros of each rank
+
+    // linearised layout: when use_linear is set, spMTTKRPs run as one pass
+    // over the Z-order copy instead of the per-mode copies
+    bool         use_linear = false;
+    dyn::LinearCopy lin;
 
     // device properties
     int          num_cus = 256;
@@ -259,6 +283,26 @@ int reject_streamed(dyn_gpu_ctx* c, const char* what);
 void stream_release(dyn_gpu_ctx* c);
 // hipFree every block parked by the allocation cache
 void flush_alloc_cache(dyn_gpu_ctx* c);
+
+// Sorting and gather building blocks (dyn_remap.hip).  sort_pairs: stable
+// LSD radix sort of (k0, v0) over the low `bits` bits, ping-ponging with
+// (k1, v1); *which (in/out) names the buffer pair holding the data.
+int sort_pairs(dyn_gpu_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+               int bits, int* which);
+int iota_u32(dyn_gpu_ctx* c, uint32_t* p, uint32_t n);
+int gather_u32(dyn_gpu_ctx* c, const uint32_t* src, const uint32_t* perm, uint32_t* dst,
+               uint32_t n);
+int gather_f32(dyn_gpu_ctx* c, const float* src, const uint32_t* perm, float* dst, uint32_t n);
+// dst[p] = relab[src[perm[p]]]
+int gather_relabel(dyn_gpu_ctx* c, const uint32_t* src, const uint32_t* perm,
+                   const uint32_t* relab, uint32_t* dst, uint32_t n);
+unsigned grid_for(uint64_t n);
+
+// Linearised layout (dyn_linear.hip): build the Z-order copy; the all-modes
+// pass writes Yhat[n] (input row order) for every mode n in `mask`.
+int build_linear(dyn_gpu_ctx* c);
+void free_linear(dyn_gpu_ctx* c);
+int linear_sweep(dyn_gpu_ctx* c, uint32_t mask);
 
 int forward_eval(dyn_gpu_ctx* c, const float* const* A, int Rp,
                  const double* lambda_or_null, float* m_out, const float* vals_dot,

@@ -303,7 +303,7 @@ static int bits_for(uint32_t dim) {
     return b;
 }
 
-static unsigned grid_for(uint64_t n) {
+unsigned grid_for(uint64_t n) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, 16384));
 }
 
@@ -327,6 +327,46 @@ static int radix_passes(dyn_gpu_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1
     }
     *which = cur;
     return DYN_OK;
+}
+
+int sort_pairs(dyn_gpu_ctx* c, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+               int bits, int* which) {
+    if (n == 0 || bits <= 0) return DYN_OK;
+    const uint32_t nblocks = (n + kRadixTile - 1) / kRadixTile;
+    const uint64_t ncounts = (uint64_t)std::max<uint32_t>(nblocks, 1) * 256;
+    void* sp2 = nullptr;
+    DYN_CHECK(scratch2(c, (ncounts + ncounts / kScanTile + 16) * sizeof(uint32_t), &sp2));
+    uint32_t* counts = (uint32_t*)sp2;
+    return radix_passes(c, k0, v0, k1, v1, n, bits, counts, counts + ncounts, which);
+}
+
+int iota_u32(dyn_gpu_ctx* c, uint32_t* p, uint32_t n) {
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, n);
+    return check_launch(c, "iota");
+}
+
+int gather_u32(dyn_gpu_ctx* c, const uint32_t* src, const uint32_t* perm, uint32_t* dst,
+               uint32_t n) {
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, perm,
+                       dst, n);
+    return check_launch(c, "gather_u32");
+}
+
+int gather_f32(dyn_gpu_ctx* c, const float* src, const uint32_t* perm, float* dst, uint32_t n) {
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_gather_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, perm,
+                       dst, n);
+    return check_launch(c, "gather_f32");
+}
+
+int gather_relabel(dyn_gpu_ctx* c, const uint32_t* src, const uint32_t* perm,
+                   const uint32_t* relab, uint32_t* dst, uint32_t n) {
+    if (n == 0) return DYN_OK;
+    hipLaunchKernelGGL(k_gather_relabel, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, src, perm,
+                       relab, dst, n);
+    return check_launch(c, "gather_relabel");
 }
 
 int count_rows(dyn_gpu_ctx* c, const uint32_t* ix, uint64_t n, uint32_t dim, uint32_t* cnt) {
@@ -362,6 +402,9 @@ int relabel_from_counts(dyn_gpu_ctx* c, int w, const uint32_t* cnt) {
     std::stable_sort(inv.begin(), inv.end(), [&](uint32_t a, uint32_t b) { return h[a] > h[b]; });
     std::vector<uint32_t> rank(dim);
     for (uint32_t r = 0; r < dim; ++r) rank[inv[r]] = r;
+    std::vector<uint32_t>& rc = c->rank_count_host[w];
+    rc.resize(dim);
+    for (uint32_t r = 0; r < dim; ++r) rc[r] = h[inv[r]];
     DYN_HIP(c, hipMemcpyAsync(c->relab[w], rank.data(), (size_t)dim * 4, hipMemcpyHostToDevice,
                               c->stream));
     DYN_HIP(c, hipStreamSynchronize(c->stream));

@@ -212,6 +212,19 @@ class GpuContext:
         self._check(self.lib.dyn_gpu_mttkrp_all(self.h, precision, arr, self.R), "mttkrp_all")
         return outs
 
+    def build_linear(self, want_perm: bool = True):
+        """Build the Z-order copy; returns its permutation (Z-order position ->
+        input position) when want_perm."""
+        perm = np.zeros(max(self.nnz, 1), dtype=np.uint32) if want_perm else None
+        self._check(self.lib.dyn_gpu_build_linear(self.h, _u32ptr(perm) if want_perm else None),
+                    "build_linear")
+        return perm[: self.nnz] if want_perm else None
+
+    def set_layout(self, layout: int) -> None:
+        """DYN_LAYOUT_COPIES (per-mode copies) or DYN_LAYOUT_LINEAR (one
+        Z-order copy, single-pass all-modes spMTTKRP)."""
+        self._check(self.lib.dyn_gpu_set_layout(self.h, int(layout)), "set_layout")
+
     def gram(self, mode: int) -> np.ndarray:
         G = np.zeros((self.R, self.R), dtype=np.float64)
         self._check(self.lib.dyn_gpu_gram(self.h, mode, G.ctypes.data_as(L._f64p)), "gram")
@@ -255,6 +268,21 @@ class GpuContext:
                         mttkrp_time_s=r.mttkrp_time_s, linalg_time_s=r.linalg_time_s,
                         remap_time_s=r.remap_time_s, fit_history=list(hist[:k]),
                         iter_time_s=list(itim[:k]), lambda_=self.get_lambda())
+
+
+def linear_plan(dims):
+    """(mask, bits, shift, key_bits) of the Z-order key for mode sizes `dims`
+    (needs no device)."""
+    lib = L.load_library()
+    N = len(dims)
+    d = np.asarray(dims, dtype=np.uint32)
+    mask = (C.c_uint64 * N)()
+    bits = (C.c_uint8 * N)()
+    shift = (C.c_uint8 * N)()
+    total = lib.dyn_linear_plan(N, _u32ptr(d), mask, bits, shift)
+    if total < 0:
+        raise ValueError(f"invalid mode sizes {dims}")
+    return list(mask), list(bits), list(shift), int(total)
 
 
 def processed_info(path):

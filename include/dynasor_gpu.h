@@ -203,14 +203,47 @@ int dyn_gpu_mttkrp(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_out,
 int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* ctx, int mode, int precision, float* Yhat_dev,
                            int ld);
 /* Every Yhat_n from one snapshot of the factors (Jacobi all-modes
- * semantics): one segmented spMTTKRP launch per mode over that mode's sorted
- * copy, all reading the same factors.  Not a single tensor stream: a stream
- * over one copy would have to scatter every other mode's contributions with
- * atomics, which measure 30-80x slower than the gathers they replace on this
- * GPU.  Yhat_out may be NULL, or an array of N host pointers (entries may be
- * NULL). */
+ * semantics).  With the per-mode copies (the default layout): one segmented,
+ * atomic-free spMTTKRP launch per mode over that mode's sorted copy, all
+ * reading the same factors.  With the linearised layout
+ * (dyn_gpu_set_layout): one pass over the single Z-order copy that scatters
+ * every mode's rows (FP32 atomics outside the hottest rows).  Yhat_out may be
+ * NULL, or an array of N host pointers (entries may be NULL). */
 int dyn_gpu_mttkrp_all(dyn_gpu_ctx* ctx, int precision, float* const* Yhat_out,
                        int ld);
+
+/* ---- linearised (Z-order) layout (replaces the Morton / ALTO layout and its
+ *      all-modes kernel: build_morton_layout, spmttkrp_all_modes_morton,
+ *      morton_build_plan, include/dynasor_morton.h, src/dynasor_morton.cpp:
+ *      103-138, 143-360, 398-579) ------------------------------------------- */
+#define DYN_LAYOUT_COPIES      0   /* one sorted copy per mode (default) */
+#define DYN_LAYOUT_LINEAR      1   /* one Z-order copy, single-pass all-modes */
+/* Key plan of the Z-order copy, no device needed: bits[n] = ceil(log2 dims[n])
+ * (at least 1); mask[n] = the key bit positions of mode n's rank, assigned
+ * LSB-first round robin over the modes with bits left (the reference's
+ * morton_build_plan); shift[n] = low rank bits left out of the key, nonzero
+ * only when the bits exceed 64 (the largest modes give up bits until they
+ * fit; the reference refuses such tensors).  Returns the key's bit count
+ * (1..64), or -1 for invalid arguments (N outside 1..8, a zero dimension or
+ * a null pointer). */
+int dyn_linear_plan(int num_modes, const uint32_t* dims, uint64_t* mask, uint8_t* bits,
+                    uint8_t* shift);
+/* Build the Z-order copy of the uploaded tensor: the nonzeros sorted by the
+ * key of their frequency ranks (a stable device radix sort), with one rank
+ * column per mode.  perm_out (optional, host nnz): Z-order position -> input
+ * position. */
+int dyn_gpu_build_linear(dyn_gpu_ctx* ctx, uint32_t* perm_out);
+/* Select the layout the spMTTKRPs and CP-ALS run on.  DYN_LAYOUT_LINEAR
+ * builds the Z-order copy if needed and releases the per-mode copies (they
+ * are rebuilt on demand after switching back).  On the linear layout
+ * dyn_gpu_mttkrp and dyn_gpu_mttkrp_all run the all-modes pass (restricted
+ * to the requested modes) with FP32 gathers whatever the precision argument
+ * (the pass is bound by its scattered FP32 additions, not by its gathers),
+ * and dyn_gpu_cpals runs Jacobi (a Gauss-Seidel request runs Jacobi with a
+ * warning, as the reference's Morton path does, src/dynasor_cpals.cpp:
+ * 232-240).  Results vary in the last bits from run to run (atomic
+ * summation order).  Not available for streamed tensors (DYN_ERR_STATE). */
+int dyn_gpu_set_layout(dyn_gpu_ctx* ctx, int layout);
 /* G = Y_mode^T Y_mode (R x R, row-major, FP64 result of FP32 MFMA partials). */
 int dyn_gpu_gram(dyn_gpu_ctx* ctx, int mode, double* G);
 
