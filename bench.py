@@ -231,14 +231,37 @@ def kernel_figures(dims, nnz, R, jacobi, steps, launches, kms, bf, chunk_nnz=0):
             "bytes_per_launch": total / launches if launches else None}
 
 
+def linear_figures(dims, nnz, R, steps, launches, kms):
+    """Algorithmic bytes and flops of the timed all-modes passes over the
+    linearised copy: each pass streams the copy once and reads and writes
+    every computed mode's rows once (B_J, FP32 rows); a Jacobi run of K
+    iterations makes K full passes plus one pass restricted to the smallest
+    mode for the last fit (its stream and every gathered mode once, one
+    output)."""
+    N = len(dims)
+    s = int(np.argmin(dims))
+    full = jacobi_pass_bytes(dims, nnz, R, 4)
+    last = nnz * (4 + 4 * N) + sum(d * R * 4 for d in dims) + dims[s] * R * 4
+    total = steps * full + last
+    if launches != steps + 1 or kms <= 0:
+        log(f"unexpected linear launch count {launches} (expected {steps + 1})")
+    flops = N * N * R * nnz * steps + N * R * nnz
+    return {"GBps": total / (kms * 1e-3) / 1e9 if kms > 0 else 0.0,
+            "GFLOPs": flops / (kms * 1e-3) / 1e9 if kms > 0 else 0.0,
+            "ms_per_launch": kms / launches if launches else None, "launches": launches,
+            "bytes_per_launch": total / launches if launches else None}
+
+
 def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True, gs_too=False,
-              ooc_chunk=None, ooc_source="host"):
+              ooc_chunk=None, ooc_source="host", layout="copies", linear_too=False):
     """Plain FP32 and mixed BF16-gather CP-ALS on one workload: s/iter, fits,
     equal-fit check and the spMTTKRP roofline figures of each.  `gs_too`
     adds the same comparison under Gauss-Seidel on the same copies (for a
     Jacobi workload); `ooc_chunk` streams the tensor from host memory in
     chunks of that many nonzeros (0: the engine default) instead of keeping
-    it resident."""
+    it resident; layout "linear" runs Jacobi on the single Z-order copy (FP32
+    gathers only); `linear_too` adds that run after the others, on the same
+    tensor and initial factors."""
     from dynasor_plus_amd import GpuContext, host, init_factors
     from dynasor_plus_amd import _lib as L
     t0 = time.time()
@@ -255,17 +278,20 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True, gs_too=Fal
            "kernel": wl["kernel"], "steps": steps, "warmup": warmup}
     legs = (("mixed", L.DYN_PREC_BF16, 2), ("plain", L.DYN_PREC_FP32, 4))
 
-    def run_pair(ctx, kern_id, is_jacobi, tag):
+    def run_pair(ctx, kern_id, is_jacobi, tag, mixed=mixed):
         runs = {}
         for label, prec, bf in (legs if mixed else legs[1:]):
             res, elapsed, launches, kms = timed_run(ctx, Y0, R, kern_id, prec, warmup, steps)
-            k = kernel_figures(dims, T.nnz, R, is_jacobi, steps, launches, kms, bf,
-                               chunk_nnz=0 if ooc_chunk is None else
-                               (ooc_chunk or min(T.nnz, 1 << 27)))
+            if layout == "linear":
+                k = linear_figures(dims, T.nnz, R, steps, launches, kms)
+            else:
+                k = kernel_figures(dims, T.nnz, R, is_jacobi, steps, launches, kms, bf,
+                                   chunk_nnz=0 if ooc_chunk is None else
+                                   (ooc_chunk or min(T.nnz, 1 << 27)))
             k["s_per_iter"] = elapsed / steps
             k["final_fit"] = res.final_fit
             k["fits"] = [round(f, 7) if math.isfinite(f) else None for f in res.fit_history]
-            if is_jacobi and ooc_chunk is None:
+            if is_jacobi and ooc_chunk is None and layout != "linear":
                 # all-modes pass over one copy as the yardstick
                 bj = jacobi_pass_bytes(dims, T.nnz, R, bf)
                 k["jacobi_pass_frac"] = bj / (kms * 1e-3 / (launches / N)) / 1e9 / HBM_PEAK_GBS
@@ -290,7 +316,12 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True, gs_too=Fal
 
     with GpuContext(0) as ctx:
         t0 = time.time()
-        if ooc_chunk is None:
+        if layout == "linear":
+            ctx.upload(T)
+            ctx.set_layout(L.DYN_LAYOUT_LINEAR)
+            out["linear_build_s"] = time.time() - t0
+            out["layout"] = "linear (one Z-order copy, single-pass all-modes)"
+        elif ooc_chunk is None:
             ctx.upload(T)
             ctx.remap_all()
             out["remap_s"] = time.time() - t0
@@ -312,9 +343,16 @@ def cpals_leg(name, wl, warmup, steps, threads, want_cpu, mixed=True, gs_too=Fal
             out["source"] = "host"
             out["chunk_nnz"] = ooc_chunk or min(T.nnz, 1 << 27)
         try:
-            out.update(run_pair(ctx, kern, jacobi, ""))
+            out.update(run_pair(ctx, kern, jacobi, "", mixed=mixed and layout != "linear"))
             if gs_too and jacobi:
                 out["gauss_seidel"] = run_pair(ctx, L.DYN_CP_GAUSS_SEIDEL, False, " gauss-seidel")
+            if linear_too and ooc_chunk is None and layout != "linear":
+                t0 = time.time()
+                ctx.set_layout(L.DYN_LAYOUT_LINEAR)
+                build_s = time.time() - t0
+                layout = "linear"
+                lin = run_pair(ctx, L.DYN_CP_JACOBI, True, " linear", mixed=False)
+                out["linear"] = {"build_s": build_s, **lin["plain"]}
         finally:
             if ooc_source == "dnb" and ooc_chunk is not None:
                 # the context keeps the file open; the unlinked file stays readable
@@ -382,6 +420,9 @@ def main():
     ap.add_argument("--ooc-source", default="host", choices=("host", "dnb"),
                     help="with --ooc: stream from the host arrays, or from a .dnb file written "
                          "to a temporary directory first")
+    ap.add_argument("--layout", default="copies", choices=("copies", "linear"),
+                    help="copies: one sorted copy per mode (default); linear: one Z-order "
+                         "copy and the single-pass all-modes kernel (Jacobi, FP32 gathers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -400,12 +441,15 @@ def main():
                                                                       threads)}), flush=True)
         return
 
-    if args.ooc is not None:
-        wl = dict(wl, kernel="jacobi")   # a streamed tensor runs the all-modes update
+    linear = args.layout == "linear"
+    if args.ooc is not None or linear:
+        # a streamed or linearised tensor runs the all-modes update
+        wl = dict(wl, kernel="jacobi")
+    plain_head = args.workload == "c4" and not args.no_extra and args.ooc is None and not linear
     head = cpals_leg(args.workload, wl, args.warmup, steps, threads,
-                     not args.no_cpu and args.ooc is None, mixed=not args.no_mixed,
-                     gs_too=args.workload == "c4" and not args.no_extra and args.ooc is None,
-                     ooc_chunk=args.ooc, ooc_source=args.ooc_source)
+                     not args.no_cpu and args.ooc is None, mixed=not args.no_mixed and not linear,
+                     gs_too=plain_head, ooc_chunk=args.ooc, ooc_source=args.ooc_source,
+                     layout=args.layout, linear_too=plain_head)
     keys = ("s_per_iter", "final_fit", "GBps", "GFLOPs", "ms_per_launch")
 
     def summary(leg):
@@ -415,7 +459,7 @@ def main():
         return d
 
     extra = {}
-    if args.workload == "c4" and not args.no_extra and args.ooc is None:
+    if args.workload == "c4" and not args.no_extra and args.ooc is None and not linear:
         for name in ("c2", "c3"):
             w = WORKLOADS[name]
             leg = cpals_leg(name, w, args.warmup, w["steps"], threads, False)
@@ -442,14 +486,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 gathered operands" if args.no_mixed else "bf16 gathered operands") +
+        "dtype": ("f32 gathered operands" if args.no_mixed or linear else "bf16 gathered operands") +
                  ", f32 accumulate, f64 Cholesky/lambda/fit",
         "data": "synthetic (engine generator, distinct coordinates); N(0,1) initial factors, seed 42",
         "config": {"workload": head["workload"], "dims": head["dims"], "nnz": head["nnz"],
                    "rank": head["rank"], "kernel": head["kernel"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": m["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["GBps"] / HBM_PEAK_GBS,
-                     "traffic": None if args.ooc is not None else
+                     "traffic": None if args.ooc is not None or linear else
                      profile_figure(tr_file, "traffic_bytes_per_launch")},
         "spmttkrp_GFLOPs": m["GFLOPs"],
         "spmttkrp_ms_per_launch": m["ms_per_launch"],
@@ -465,6 +509,9 @@ def main():
                   "jacobi_pass_frac": m.get("jacobi_pass_frac"),
                   "plain_jacobi_pass_frac": head["plain"].get("jacobi_pass_frac")},
         "gauss_seidel": summary(head["gauss_seidel"]) if "gauss_seidel" in head else None,
+        "linear": ({k: head["linear"][k] for k in ("build_s",) + keys + ("fits",)}
+                   if "linear" in head else None),
+        "layout": head.get("layout", "copies (one sorted copy per mode)"),
         "streamed": ({"chunk_nnz": head["chunk_nnz"], "attach_s": head["attach_s"],
                       "source": head["source"], "stream_GBps": m.get("stream_GBps")}
                      if args.ooc is not None else None),

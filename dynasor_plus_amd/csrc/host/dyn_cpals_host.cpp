@@ -118,6 +118,12 @@ CpResult cpals(Tensor& T, FactorMatrices& F, const CpOptions& opt) {
         if (opt.verbose)
             std::printf("[io] wrote processed state to %s\n", opt.save_processed_path.c_str());
     }
+    if (T.layout == Layout::Morton && opt.stream_path.empty() && !opt.stream_from_host) {
+        // the linearised copy replaces the per-mode copies (all-modes only,
+        // src/dynasor_cpals.cpp:232-240)
+        rc = dyn_gpu_set_layout(ctx.h, DYN_LAYOUT_LINEAR);
+        if (rc != DYN_OK) return fail("dyn_gpu_set_layout", rc);
+    }
     rc = dyn_gpu_set_factors(ctx.h, R, F.Y.data(), Rp);
     if (rc != DYN_OK) return fail("dyn_gpu_set_factors", rc);
 

@@ -5,9 +5,11 @@
 //   (no --decompose)           one spMTTKRP sweep over every mode, with
 //                              --verify / --dump / --dump-bin / --all-modes, so
 //                              the reference's scripts/run_tests.sh can drive it
-// CPU-layout flags of the reference (--layout, --ncopy-*, --shard-size, --jit,
-// --ooc*, --threads) are accepted and ignored; --save-processed / --runtime
-// write and read the device's processed state (dynasor_dnp.h).
+// --layout morton selects the linearised Z-order copy and --ooc / --layout ooc
+// the host-streamed tensor; the CPU-only layout flags of the reference
+// (--layout pingpong|ncopy|inplace, --ncopy-*, --shard-size, --jit,
+// --threads) are accepted and ignored; --save-processed / --runtime write and
+// read the device's processed state (dynasor_dnp.h).
 #include "dynasor_common.h"
 #include "dynasor_cpals.h"
 #include "dynasor_dnp.h"
@@ -15,6 +17,7 @@
 #include "dynasor_host.h"
 #include "dynasor_ooc.h"
 
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -79,6 +82,10 @@ void usage(const char* a0) {
         "  --ooc on|off|auto      stream the tensor from host memory once per sweep\n"
         "                         (Jacobi); auto: when the device copies do not fit\n"
         "  --ooc-chunk NNZ        streaming chunk in nonzeros (default 2^27)\n"
+        "  --layout L             auto | morton (one linearised Z-order copy, all-modes\n"
+        "                         pass; auto picks it when the per-mode copies do not fit\n"
+        "                         but one copy does) | ooc; CPU layouts are accepted and\n"
+        "                         ignored\n"
         "  --save-processed P     write the processed device state (tensor + per-mode\n"
         "                         permutations) to P after the copies are built\n"
         "  --runtime              <tensor> is a processed-state file: load it and skip\n"
@@ -139,6 +146,13 @@ bool parse(int argc, char** argv, Args& a) {
     return true;
 }
 
+// --layout values that select the linearised copy (the reference's "morton")
+bool is_linear_layout(const std::string& layout) {
+    std::string l = layout;
+    for (char& ch : l) ch = (char)std::tolower((unsigned char)ch);
+    return l == "morton" || l == "linear" || l == "zorder" || l == "alto";
+}
+
 // spMTTKRP by definition (src/dynasor_reference.cpp:18-50), the CLI's own
 // --verify check of the device result
 void cpu_check_mttkrp(const Tensor& T, const FactorMatrices& F, int n, std::vector<float>& out) {
@@ -175,7 +189,8 @@ int mttkrp_mode_main(const Args& args, Tensor& T) {
     int rc = dyn_gpu_create(args.device, &h);
     if (rc == DYN_OK) rc = dyn_gpu_upload_coo(h, T.num_modes, T.nnz, T.mode_size, T.vals, T.idx_buf);
     if (rc == DYN_OK) rc = dyn_gpu_set_factors(h, F.rank, F.Y.data(), F.rank_padded);
-    if (rc == DYN_OK) rc = dyn_gpu_remap_all(h);
+    const bool linear = is_linear_layout(args.layout);
+    if (rc == DYN_OK) rc = linear ? dyn_gpu_set_layout(h, DYN_LAYOUT_LINEAR) : dyn_gpu_remap_all(h);
     if (rc != DYN_OK) {
         std::fprintf(stderr, "[gpu] setup failed (status %d): %s\n", rc,
                      h ? dyn_gpu_last_error(h) : "no device");
@@ -199,9 +214,9 @@ int mttkrp_mode_main(const Args& args, Tensor& T) {
         return 2;
     }
     dyn_gpu_destroy(h);
-    std::printf("[dynasor] device spMTTKRP (%s, %s) over %d modes = %.6f s\n",
-                args.all_modes ? "all-modes" : "per-mode", prec ? "bf16" : "fp32", T.num_modes,
-                secs);
+    std::printf("[dynasor] device spMTTKRP (%s, %s%s) over %d modes = %.6f s\n",
+                args.all_modes ? "all-modes" : "per-mode", (prec && !linear) ? "bf16" : "fp32",
+                linear ? ", linearised Z-order copy" : "", T.num_modes, secs);
     if (!args.dump_prefix.empty()) {
         const char* ext = args.dump_binary ? ".bin" : ".txt";
         for (int n = 0; n < T.num_modes; ++n) {
@@ -330,6 +345,12 @@ int main(int argc, char** argv) {
             std::printf("[ooc] streaming the tensor from %s in chunks of %llu nonzeros\n",
                         stream_dnb.empty() ? "host memory" : stream_dnb.c_str(),
                         (unsigned long long)opt.stream_chunk_nnz);
+        } else if (cp.storage.choice == Layout::Morton) {
+            // src/main.cpp:712-733: the tensor is converted to the linearised
+            // layout, which cpals() runs with the all-modes update
+            T.layout = Layout::Morton;
+            std::printf("[layout] linearised Z-order copy (%.2f GiB on the device)\n",
+                        cp.device_linear_bytes / 1073741824.0);
         } else if (!cp.fits) {
             // --ooc off: the device engine needs the tensor and its copies resident
             std::fprintf(stderr,

@@ -72,6 +72,12 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     p.device_tensor_bytes = tb;
     const size_t sort_scratch = (size_t)T.nnz * 12;
     p.device_total_bytes = tb + dfb + sort_scratch;
+    // linearised layout: the COO upload, one Z-order copy (values, N rank
+    // columns, permutation), the rank-order accumulators and its key sort's
+    // scratch (four u32 columns)
+    p.device_linear_bytes = (size_t)T.nnz * (4 + 4 * (size_t)N) +
+                            (size_t)T.nnz * (4 + 4 * (size_t)N + 4) + acc + dfb +
+                            (size_t)T.nnz * 16;
     if (!have_dev) fr = tot = 0;
     if (have_dev) {
         p.device_free_bytes = fr;
@@ -101,10 +107,24 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
         osrc = "env-ooc";
     }
     const bool ooc_forced = oc == "on" || lc == "ooc" || lc == "outofcore" || lc == "out-of-core";
+    const bool linear_req = lc == "morton" || lc == "linear" || lc == "zorder" || lc == "alto";
+    const bool linear_fits = !have_dev || p.device_linear_bytes < fr;
+    sp.inplace_bytes = p.device_linear_bytes;
+    sp.inplace_need = p.device_linear_bytes;
     if (ooc_forced) {
         sp.choice = Layout::OutOfCore;
         sp.reason = "streamed from host memory (requested)";
         sp.override_source = oc == "on" ? osrc : "cli-layout";
+    } else if (linear_req) {
+        sp.choice = Layout::Morton;
+        sp.reason = "one linearised (Z-order) copy (requested)";
+        sp.override_source = "cli-layout";
+    } else if (have_dev && !p.fits && linear_fits) {
+        // src/dynasor_preprocess.cpp:951-1104 picks the layout the memory
+        // allows; on the device one linearised copy sits between the per-mode
+        // copies and streaming from host memory
+        sp.choice = Layout::Morton;
+        sp.reason = "per-mode sorted copies exceed free device memory; one linearised copy fits";
     } else if (have_dev && !p.fits && oc != "off") {
         sp.choice = Layout::OutOfCore;
         sp.reason = "per-mode sorted copies exceed free device memory: streamed from host memory";
@@ -113,7 +133,10 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     }
     // two chunk buffers, one sorted copy and the sort scratch of a chunk
     sp.ooc_need = (size_t)std::min<uint64_t>(T.nnz, 1ull << 27) * (4 + 4 * (size_t)N) * 4 + dfb;
-    const bool ooc = sp.choice == Layout::OutOfCore;
+    // out-of-core and the linearised copy run the all-modes update only
+    const bool ooc = sp.choice == Layout::OutOfCore || sp.choice == Layout::Morton;
+    if (sp.choice == Layout::Morton) p.layout = "device-zorder";
+    else if (sp.choice == Layout::OutOfCore) p.layout = "host-stream";
 
     // ---- kernel: the reference's decision tree ---------------------------
     // (src/dynasor_preprocess.cpp:1253-1308) with the all-modes pass's
@@ -200,8 +223,13 @@ void dyn_jit_force_enable(bool on) {
 
 void dyn_print_compute_plan(const ComputePlan& p, const char* prefix) {
     const double GiB = 1024.0 * 1024.0 * 1024.0;
-    std::printf("%s layout : %s (one sorted copy per mode on the device; reason=%s; src=%s)\n",
-                prefix, p.layout, p.storage.reason, p.storage.override_source);
+    const char* what = p.storage.choice == Layout::Morton      ? "one Z-order copy on the device"
+                       : p.storage.choice == Layout::OutOfCore ? "streamed from host memory"
+                                                               : "one sorted copy per mode on the device";
+    std::printf("%s layout : %s (%s; reason=%s; src=%s)\n", prefix, p.layout, what,
+                p.storage.reason, p.storage.override_source);
+    std::printf("%s linear : %.2f GiB for one linearised copy\n", prefix,
+                p.device_linear_bytes / GiB);
     std::printf("%s device : tensor=%.2f GiB  factors=%.2f GiB  total=%.2f GiB  free=%.2f GiB%s\n",
                 prefix, p.device_tensor_bytes / GiB, p.device_factor_bytes / GiB,
                 p.device_total_bytes / GiB, p.device_free_bytes / GiB,

@@ -26,15 +26,17 @@ using idx_t = uint32_t;    // include/dynasor_common.h:28
 using value_t = float;     // include/dynasor_common.h:30
 
 // Storage layouts (include/dynasor_common.h:113-165).  The device engine
-// always holds the tensor as one sorted copy per mode in device memory
-// (DeviceCopies); the CPU layouts keep their values so callers that read,
-// compare or assign a layout compile and behave unchanged.
+// holds the tensor as one sorted copy per mode in device memory
+// (DeviceCopies), as one linearised Z-order copy (Morton: the reference's
+// ALTO-style layout, single-pass all-modes, Jacobi only) or streams it from
+// host memory (OutOfCore); the CPU-only layouts keep their values so callers
+// that read, compare or assign a layout compile and behave unchanged.
 enum class Layout : uint8_t {
     PingPong = 0,
     NCopy = 1,
     InPlace = 2,
     OutOfCore = 3,
-    Morton = 4,
+    Morton = 4,         // one linearised (Z-order) copy in HBM (dyn_gpu_set_layout)
     DeviceCopies = 5,   // per-mode sorted copies in HBM (dyn_gpu_remap)
 };
 
@@ -131,7 +133,8 @@ struct ComputePlan {
     size_t      device_factor_bytes = 0;  // Y, BF16 shadow, Yhat on the device
     size_t      device_total_bytes = 0;
     size_t      device_free_bytes = 0;    // at planning time (0 if unknown)
-    bool        fits = true;
+    size_t      device_linear_bytes = 0;  // COO + one Z-order copy + accumulators + factors
+    bool        fits = true;              // per-mode copies fit in free device memory
     const char* layout = "device-remap";
 };
 

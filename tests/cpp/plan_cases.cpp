@@ -2,7 +2,8 @@
 // budget, for tests/test_plan.py:
 //   plan_cases <nnz> <rank> <kernel_cli> <have_device 0|1> <free_bytes> <total_bytes> <dims...>
 // Output (tab-separated): "kernel=<0|1> reason=<...> source=<...> fits=<0|1> ofibs=<bytes>
-//          need=<bytes> choice=<layout>"
+//          need=<bytes> choice=<layout> linear=<bytes> layout=<name>"
+// (environment PLAN_LAYOUT: the --layout value, default auto)
 #include "dynasor_common.h"
 
 #include <cstdio>
@@ -21,10 +22,12 @@ int main(int argc, char** argv) {
     const uint64_t tot = std::strtoull(argv[6], nullptr, 10);
     T.num_modes = argc - 7;
     for (int n = 0; n < T.num_modes; ++n) T.mode_size[n] = (idx_t)std::strtoul(argv[7 + n], nullptr, 10);
-    ComputePlan p = dyn_build_compute_plan_with_memory(T, rank, 1, "auto", "auto", kernel, have,
-                                                       fr, tot);
-    std::printf("kernel=%d\treason=%s\tsource=%s\tfits=%d\tofibs=%zu\tneed=%zu\tchoice=%d\n",
+    const char* layout = std::getenv("PLAN_LAYOUT");
+    ComputePlan p = dyn_build_compute_plan_with_memory(T, rank, 1, layout ? layout : "auto", "auto",
+                                                       kernel, have, fr, tot);
+    std::printf("kernel=%d\treason=%s\tsource=%s\tfits=%d\tofibs=%zu\tneed=%zu\tchoice=%d\t"
+                "linear=%zu\tlayout=%s\n",
                 p.kernel_enum, p.kernel_reason, p.kernel_source, (int)p.fits, p.ofibs_bytes,
-                p.device_total_bytes, (int)p.storage.choice);
+                p.device_total_bytes, (int)p.storage.choice, p.device_linear_bytes, p.layout);
     return 0;
 }

@@ -96,3 +96,42 @@ def test_cli_all_modes_and_bf16(tmp_path):
     # BF16 gathers: within the reference's own 1e-4 tolerance? not guaranteed;
     # the run must complete and report a comparison
     assert rc in (0, 3) and "[compare] worst" in out, out
+
+
+def test_cli_morton_layout(tmp_path):
+    """--layout morton: the standalone sweep on the linearised copy passes the
+    reference's --verify check, and --decompose cpals runs Jacobi on it (a
+    Gauss-Seidel request is promoted) with the same fits as Jacobi on the
+    per-mode copies, since both compute every spMTTKRP from one snapshot.  On
+    this tensor Jacobi does not converge and amplifies summation-order
+    rounding (the linear pass adds with atomics) from the third iteration on,
+    so the trajectories are compared over the first two."""
+    say("cli morton")
+    z = golden("verify_g5_r32.npz")
+    T = tensor_from_npz(z)
+    p = tmp_path / "t.tns"
+    tio.save_tns(p, T, fmt="%.9g")
+    rc, out = run(p, "--rank", 32, "--seed", int(z["seed"]), "--verify", "--all-modes",
+                  "--layout", "morton", "--no-cache", "--dump", tmp_path / "o", "--dump-bin")
+    assert rc == 0 and re.search(r"^PASS:", out, re.M), out
+    assert "linearised Z-order copy" in out, out
+    for n in range(T.num_modes):
+        got = tio.load_dyfm(tmp_path / f"o.fm{n}_out.bin")
+        ref = z[f"yhat{n}"]
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    s = golden("cpals_synth_cp_r5_gauss-seidel_r5.npz")
+    Ts = tensor_from_npz(s)
+    q = tmp_path / "s.tns"
+    tio.save_dnb(str(q) + ".dnb", Ts)
+    rc, out = run(q, "--decompose", "cpals", "--rank", 5, "--cp-iters", 12, "--cp-tol", 0,
+                  "--layout", "morton", "--cp-kernel", "gauss-seidel")
+    assert rc == 0, out
+    assert "device-zorder" in out and "[layout] linearised" in out, out
+    assert "user-wanted-gs-but-layout-forces-jacobi" in out, out
+    lin = np.array([float(m.group(2)) for m in FIT.finditer(out)])
+    rc, out = run(q, "--decompose", "cpals", "--rank", 5, "--cp-iters", 12, "--cp-tol", 0,
+                  "--cp-kernel", "jacobi")
+    assert rc == 0, out
+    cop = np.array([float(m.group(2)) for m in FIT.finditer(out)])
+    assert len(lin) == len(cop) == 12
+    np.testing.assert_allclose(lin[:2], cop[:2], rtol=0, atol=2e-6)

@@ -133,11 +133,37 @@ def test_out_of_core_choice(plan_exe, env, cli, kernel, reason, choice):
     assert (int(out["kernel"]), out["reason"], int(out["choice"])) == (kernel, reason, choice), out
 
 
-def test_out_of_core_when_copies_do_not_fit(plan_exe):
+def test_memory_tiers_copies_linear_stream(plan_exe):
+    """The per-mode copies when they fit; else one linearised copy when that
+    fits (Layout::Morton, Jacobi); else streaming from host memory unless
+    out-of-core is off, in which case the plan does not fit."""
     dims = (12092, 9184, 28818)
     out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims)
-    need = int(out["need"])
+    need, lin = int(out["need"]), int(out["linear"])
+    assert int(out["choice"]) == 5 and out["layout"] == "device-remap"
+    assert lin < need
     out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=need - 1)
-    assert int(out["choice"]) == 3 and int(out["kernel"]) == 1
+    assert (int(out["choice"]), int(out["kernel"]), out["layout"]) == (4, 1, "device-zorder"), out
+    assert out["reason"] == "forced-jacobi-by-layout"
     out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=need - 1, env={"DYN_OOC": "off"})
+    assert int(out["choice"]) == 4
+    out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=lin - 1)
+    assert (int(out["choice"]), int(out["kernel"]), out["layout"]) == (3, 1, "host-stream"), out
+    out, _ = plan(plan_exe, 77_000_000, 32, "auto", dims, free=lin - 1, env={"DYN_OOC": "off"})
     assert int(out["choice"]) == 5 and int(out["fits"]) == 0
+
+
+@pytest.mark.parametrize("layout", ["morton", "linear", "MORTON"])
+@pytest.mark.parametrize("cli,reason", [("auto", "forced-jacobi-by-layout"),
+                                        ("gs", "user-wanted-gs-but-layout-forces-jacobi"),
+                                        ("jacobi", "user-forced-jacobi")])
+def test_linear_layout_request(plan_exe, layout, cli, reason):
+    """--layout morton selects the linearised copy, which runs Jacobi only, as
+    the reference's Morton layout does (src/dynasor_preprocess.cpp:1254-1300)."""
+    out, _ = plan(plan_exe, 2_000_000, 32, cli, (1000, 1000, 1000), env={"PLAN_LAYOUT": layout})
+    assert (int(out["choice"]), int(out["kernel"]), out["reason"]) == (4, 1, reason), out
+
+
+def test_cpu_layouts_are_ignored(plan_exe):
+    out, _ = plan(plan_exe, 2_000_000, 32, "auto", (1000, 1000, 1000), env={"PLAN_LAYOUT": "ncopy"})
+    assert int(out["choice"]) == 5 and int(out["kernel"]) == 1
