@@ -5,7 +5,11 @@
 #include "dynasor_host.h"
 #include "dynasor_ooc.h"
 
+#include <fcntl.h>
+#include <omp.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cctype>
@@ -204,6 +208,213 @@ static bool load_tns_text(const std::string& path, Tensor& T) {
 }
 
 // ---------------------------------------------------------------------------
+//  FROSTT text, parallel (the reference's load_tns_mmap,
+//  src/dynasor_io.cpp:960-1176): the file is mapped, cut into one slice of
+//  whole lines per thread, every slice's data lines are counted, and each
+//  thread then parses its slice straight into the tensor at its offset.
+//  Indices must be plain decimal integers and every data line must carry
+//  N + 1 tokens; a file that is not of that shape (an index written as 3.0,
+//  a ragged line) returns false so that the caller falls back to the serial
+//  parser above, which accepts it or reports the error.  Results are those
+//  of the serial parser bit for bit: the same double parse of each value
+//  cast to float, and the mode sizes are the larger of a SPLATT header's
+//  and the largest index seen.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Mapped {
+    const char* p = nullptr;
+    size_t n = 0;
+    ~Mapped() {
+        if (p) ::munmap((void*)p, n);
+    }
+};
+
+inline bool blank(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// first byte of the line after `pos` (or n)
+inline size_t next_line(const char* b, size_t pos, size_t n) {
+    const void* q = pos < n ? std::memchr(b + pos, '\n', n - pos) : nullptr;
+    return q ? (size_t)((const char*)q - b) + 1 : n;
+}
+
+// Tokens of one line as doubles (the serial parser's reading), at most
+// max_tok; returns the count, or -1 if a token does not parse.
+int line_tokens(const char* s, const char* e, double* tok, int max_tok) {
+    char local[512];
+    const size_t len = std::min<size_t>((size_t)(e - s), sizeof(local) - 1);
+    std::memcpy(local, s, len);
+    local[len] = 0;
+    int nt = 0;
+    char* p = local;
+    while (*p) {
+        while (*p && std::isspace((unsigned char)*p)) ++p;
+        if (!*p) break;
+        if (nt >= max_tok) return -1;
+        char* end = nullptr;
+        const double v = std::strtod(p, &end);
+        if (end == p) return -1;
+        tok[nt++] = v;
+        p = end;
+    }
+    return nt;
+}
+
+// Whether [s, e) is a data line: not blank, not a comment.
+inline bool data_line(const char* s, const char* e) {
+    while (s < e && (blank(*s) || *s == '\n')) ++s;
+    return s < e && *s != '#';
+}
+
+}  // namespace
+
+static bool load_tns_parallel(const std::string& path, Tensor& T, int* threads_used) {
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return false;
+    struct stat st {};
+    if (::fstat(fd, &st) != 0 || st.st_size <= 0) {
+        ::close(fd);
+        return false;
+    }
+    Mapped m;
+    m.n = (size_t)st.st_size;
+    void* map = ::mmap(nullptr, m.n, PROT_READ, MAP_PRIVATE, fd, 0);
+    ::close(fd);
+    if (map == MAP_FAILED) return false;
+    m.p = (const char*)map;
+    (void)::madvise(map, m.n, MADV_SEQUENTIAL);
+    const char* b = m.p;
+    const size_t n = m.n;
+
+    // leading blank / comment lines, an optional SPLATT header ("N", then the
+    // N mode sizes), and the first data line's token count
+    size_t pos = 0;
+    double tok[DYN_MAX_MODES + 2];
+    int nm = 0;
+    idx_t hdr[DYN_MAX_MODES] = {0};
+    auto next_data = [&](size_t& p) {
+        while (p < n) {
+            const size_t e = next_line(b, p, n);
+            if (data_line(b + p, b + e)) return true;
+            p = e;
+        }
+        return false;
+    };
+    if (!next_data(pos)) return false;
+    size_t e0 = next_line(b, pos, n);
+    int nt = line_tokens(b + pos, b + e0, tok, DYN_MAX_MODES + 1);
+    if (nt == 1 && tok[0] == (double)(int)tok[0] && tok[0] >= 2 && tok[0] <= DYN_MAX_MODES) {
+        nm = (int)tok[0];
+        pos = e0;
+        if (!next_data(pos)) return false;
+        e0 = next_line(b, pos, n);
+        if (line_tokens(b + pos, b + e0, tok, DYN_MAX_MODES + 1) != nm) return false;
+        for (int k = 0; k < nm; ++k) hdr[k] = (idx_t)tok[k];
+        pos = e0;
+        if (!next_data(pos)) return false;
+        e0 = next_line(b, pos, n);
+        nt = line_tokens(b + pos, b + e0, tok, DYN_MAX_MODES + 1);
+        if (nt != nm + 1) return false;
+    } else {
+        if (nt < 2 || nt > DYN_MAX_MODES + 1) return false;
+        nm = nt - 1;
+    }
+    const size_t start = pos;
+
+    // one slice of whole lines per thread (at least 1 MiB each)
+    int nth = std::max(1, omp_get_max_threads());
+    nth = (int)std::min<size_t>((size_t)nth, std::max<size_t>(1, (n - start) >> 20));
+    std::vector<size_t> cut(nth + 1, n);
+    cut[0] = start;
+    for (int t = 1; t < nth; ++t)
+        cut[t] = std::max(cut[t - 1], next_line(b, start + (n - start) / nth * t - 1, n));
+    std::vector<uint64_t> count(nth + 1, 0);
+#pragma omp parallel for num_threads(nth) schedule(static, 1)
+    for (int t = 0; t < nth; ++t) {
+        uint64_t c = 0;
+        for (size_t p = cut[t]; p < cut[t + 1];) {
+            const size_t e = next_line(b, p, cut[t + 1]);
+            c += data_line(b + p, b + e) ? 1 : 0;
+            p = e;
+        }
+        count[t + 1] = c;
+    }
+    for (int t = 0; t < nth; ++t) count[t + 1] += count[t];
+    const uint64_t total = count[nth];
+    if (total == 0) return false;
+    if (!T.try_allocate_elements(total, nm)) return false;
+
+    std::vector<int> ok(nth, 1);
+    std::vector<idx_t> mx((size_t)nth * DYN_MAX_MODES, 0);
+#pragma omp parallel for num_threads(nth) schedule(static, 1)
+    for (int t = 0; t < nth; ++t) {
+        uint64_t k = count[t];
+        idx_t* lmx = &mx[(size_t)t * DYN_MAX_MODES];
+        for (size_t p = cut[t]; p < cut[t + 1] && ok[t];) {
+            const size_t e = next_line(b, p, cut[t + 1]);
+            const char* s = b + p;
+            const char* le = b + e;
+            p = e;
+            if (!data_line(s, le)) continue;
+            for (int w = 0; w < nm; ++w) {
+                while (s < le && blank(*s)) ++s;
+                uint64_t v = 0;
+                const char* d = s;
+                while (s < le && *s >= '0' && *s <= '9' && s - d < 12) v = v * 10 + (uint64_t)(*s++ - '0');
+                // an index must be a whole decimal number in [1, 2^32]
+                if (s == d || (s < le && !blank(*s) && *s != '\n') || v == 0 || v > 0x100000000ull) {
+                    ok[t] = 0;
+                    break;
+                }
+                const idx_t ix = (idx_t)(v - 1);
+                T.idx_buf[w][k] = ix;
+                if (ix + 1 > lmx[w]) lmx[w] = ix + 1;
+            }
+            if (!ok[t]) break;
+            while (s < le && blank(*s)) ++s;
+            // the value: the serial parser's strtod, on a terminated copy
+            char local[128];
+            size_t len = 0;
+            while (s + len < le && !blank(s[len]) && s[len] != '\n' && len + 1 < sizeof(local)) {
+                local[len] = s[len];
+                ++len;
+            }
+            local[len] = 0;
+            char* end = nullptr;
+            const double val = std::strtod(local, &end);
+            if (len == 0 || end != local + len) {
+                ok[t] = 0;
+                break;
+            }
+            // nothing but blanks may follow the value
+            for (const char* q = s + len; q < le; ++q)
+                if (!blank(*q) && *q != '\n') {
+                    ok[t] = 0;
+                    break;
+                }
+            if (!ok[t]) break;
+            T.vals[k] = (value_t)val;
+            ++k;
+        }
+        if (ok[t] && k != count[t + 1]) ok[t] = 0;
+    }
+    for (int t = 0; t < nth; ++t)
+        if (!ok[t]) {
+            T.free_elements();
+            T.num_modes = 0;
+            T.nnz = 0;
+            return false;
+        }
+    for (int w = 0; w < nm; ++w) {
+        idx_t s = hdr[w];
+        for (int t = 0; t < nth; ++t) s = std::max(s, mx[(size_t)t * DYN_MAX_MODES + w]);
+        T.mode_size[w] = s;
+    }
+    if (threads_used) *threads_used = nth;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 //  .dnb cache
 // ---------------------------------------------------------------------------
 bool save_dnb_tensor(const std::string& path, const Tensor& T) {
@@ -319,11 +530,23 @@ bool load_coo_tensor(const std::string& path, Tensor& T, bool cache_on_success) 
         std::fprintf(stderr, "[io] WARN: stale or invalid binary cache '%s', reparsing text\n",
                      dnb.c_str());
     }
-    if (!load_tns_text(path, T)) return false;
+    // the parallel parser first (src/dynasor_io.cpp:1215-1234), the serial
+    // one when it declines the file; DYN_TNS_PARSER=serial skips the former
+    int threads = 0;
+    const char* env = std::getenv("DYN_TNS_PARSER");
+    const bool parallel_ok = !(env && std::strcmp(env, "serial") == 0);
+    bool parsed = parallel_ok && load_tns_parallel(path, T, &threads);
+    if (!parsed) {
+        threads = 0;
+        if (!load_tns_text(path, T)) return false;
+    }
     std::printf("[io] loaded %llu nonzeros, %d modes, shape =", (unsigned long long)T.nnz,
                 T.num_modes);
     for (int n = 0; n < T.num_modes; ++n) std::printf(" %u", (unsigned)T.mode_size[n]);
-    std::printf("  (text)\n");
+    if (threads > 0)
+        std::printf("  (text, parallel parser on %d thread%s)\n", threads, threads > 1 ? "s" : "");
+    else
+        std::printf("  (text)\n");
     if (cache_on_success && save_dnb_tensor(dnb, T))
         std::printf("[io] wrote binary cache '%s'\n", dnb.c_str());
     return true;
