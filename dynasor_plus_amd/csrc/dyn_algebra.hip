@@ -798,7 +798,7 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
                      "per-mode copies: running Jacobi (all-modes) instead\n");
         jacobi = true;
     }
-    // the linearised pass gathers FP32 rows (it is bound by its scattered FP32
+    // the linearised pass gathers FP32 rows (it is bound by its scattered
     // additions, so BF16 operands would not shorten it)
     const bool bf = o->precision == DYN_PREC_BF16 && !c->use_linear;
 

@@ -89,8 +89,8 @@ struct LinearCopy {
     uint64_t  mask[kMaxModes] = {0};           // key bits of mode n
     uint8_t   bits[kMaxModes] = {0};           // bits of mode n's rank (>= 1)
     uint8_t   shift[kMaxModes] = {0};          // low rank bits left out of the key
-    float*    acc = nullptr;                    // rank-order accumulators, sum_n I_n x Rp
-    size_t    acc_floats = 0;
+    double*   acc = nullptr;                    // rank-order FP64 accumulators, sum_n I_n x Rp
+    size_t    acc_count = 0;
 };
 
 constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a piece

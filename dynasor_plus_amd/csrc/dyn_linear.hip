@@ -18,12 +18,15 @@
 // path), and N scattered row updates.  A scattered update goes to the wave's
 // private LDS tile when its rank is one of the mode's hottest (plain
 // read-add-write: the tile belongs to one wave, and the nonzeros of one
-// instruction are applied one after the other), else to an FP32 global
-// atomic.  Each wave adds its tiles into the rank-order accumulators at the
-// end.  The atomics make the summation order, and so the last bits of the
-// result, vary from run to run; the per-mode copies stay the deterministic
-// path.  On this GPU the atomics bound the pass: about 10 G row updates/s
-// against about 130 G row gathers/s, so the per-mode segmented passes are
+// instruction are applied one after the other), else to an FP64 global
+// atomic.  Each wave adds its tiles into the rank-order FP64 accumulators at
+// the end.  FP64 because a row that takes n scattered additions in FP32 drifts
+// by about sqrt(n) units in the last place: rows of ~180k updates were 4e-5
+// off at the 400M-nonzero 5-mode workload, against 1e-7 for the per-mode
+// copies' tree-shaped sums.  The atomics make the summation order vary from
+// run to run; the per-mode copies stay the deterministic path.  On this GPU
+// the atomics bound the pass (about 10 G FP32 or 5.8 G FP64 row updates/s
+// against about 130 G row gathers/s), so the per-mode segmented passes are
 // faster whenever their N copies fit; the linear copy needs one.
 #include "dyn_internal.hpp"
 
@@ -127,14 +130,14 @@ __global__ void k_zkey(KeyArgs a, uint32_t n, uint32_t* __restrict__ klo,
 }
 
 // out[i][col] = acc[relab[i]][col] for the I rows of one mode (input row order)
-__global__ void k_unrelabel(const float* __restrict__ acc, const uint32_t* __restrict__ relab,
+__global__ void k_unrelabel(const double* __restrict__ acc, const uint32_t* __restrict__ relab,
                             float* __restrict__ out, size_t rows, int RP) {
     const size_t n = rows * (size_t)RP;
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; e < n; e += stride) {
         const size_t r = e / RP, col = e % RP;
-        out[e] = acc[(size_t)relab[r] * RP + col];
+        out[e] = (float)acc[(size_t)relab[r] * RP + col];
     }
 }
 
@@ -145,7 +148,7 @@ struct LinArgs {
     const float* vals;
     const uint32_t* idx[kMaxModes];   // ranks
     const float* Y[kMaxModes];        // rank-order FP32 factor rows
-    float* acc[kMaxModes];            // rank-order outputs; null: mode not computed
+    double* acc[kMaxModes];           // rank-order FP64 outputs; null: mode not computed
     uint32_t hot[kMaxModes];          // ranks below hot[n] go to the wave's LDS tile
     uint32_t toff[kMaxModes];         // float offset of mode n's tile in the wave's slice
     uint32_t wave_floats;             // floats of one wave's LDS slice
@@ -224,9 +227,9 @@ __device__ inline void lin_apply(const LinArgs& a, const LinGroup<RP, N>& g, boo
         const uint32_t r = g.r[n];
         const bool hot = r < a.hot[n];
         if (valid && !hot) {
-            float* dst = a.acc[n] + (size_t)r * RP + col;
+            double* dst = a.acc[n] + (size_t)r * RP + col;
 #pragma unroll
-            for (int f = 0; f < F; ++f) unsafeAtomicAdd(dst + f, out[n][f]);
+            for (int f = 0; f < F; ++f) unsafeAtomicAdd(dst + f, (double)out[n][f]);
         }
         // the tile is the wave's own; nonzeros of one instruction that share a
         // row are applied one sub-group after the other (LDS operations of a
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kLinThreads) void k_lin(LinArgs a) {
         const uint32_t m = a.hot[n] * RP;
         for (uint32_t e = lane; e < m; e += 64) {
             const float x = tile[a.toff[n] + e];
-            if (x != 0.0f) unsafeAtomicAdd(a.acc[n] + e, x);
+            if (x != 0.0f) unsafeAtomicAdd(a.acc[n] + e, (double)x);
         }
     }
 }
@@ -431,12 +434,12 @@ int linear_sweep(dyn_gpu_ctx* c, uint32_t mask) {
         off[m] = need;
         need += (size_t)c->dims[m] * Rp;
     }
-    if (L.acc_floats < need) {
+    if (L.acc_count < need) {
         dfree(c, L.acc);
         L.acc = nullptr;
-        L.acc_floats = 0;
+        L.acc_count = 0;
         DYN_CHECK(dalloc_t(c, &L.acc, need));
-        L.acc_floats = need;
+        L.acc_count = need;
     }
     LinArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -446,7 +449,9 @@ int linear_sweep(dyn_gpu_ctx* c, uint32_t mask) {
         a.Y[m] = c->Yg[m];
         if ((mask >> m) & 1u) {
             a.acc[m] = L.acc + off[m];
-            DYN_CHECK(fill_f32(c, a.acc[m], (size_t)c->dims[m] * Rp, 0.0f));
+            // all-zero bytes are 0.0 in FP64 too
+            DYN_CHECK(fill_f32(c, reinterpret_cast<float*>(a.acc[m]), 2 * (size_t)c->dims[m] * Rp,
+                               0.0f));
         }
     }
     const unsigned grid = (unsigned)c->num_cus;

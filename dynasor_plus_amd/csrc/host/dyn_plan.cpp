@@ -73,10 +73,10 @@ ComputePlan dyn_build_compute_plan_with_memory(const Tensor& T, int rank_padded,
     const size_t sort_scratch = (size_t)T.nnz * 12;
     p.device_total_bytes = tb + dfb + sort_scratch;
     // linearised layout: the COO upload, one Z-order copy (values, N rank
-    // columns, permutation), the rank-order accumulators and its key sort's
-    // scratch (four u32 columns)
+    // columns, permutation), the rank-order FP64 accumulators and its key
+    // sort's scratch (four u32 columns)
     p.device_linear_bytes = (size_t)T.nnz * (4 + 4 * (size_t)N) +
-                            (size_t)T.nnz * (4 + 4 * (size_t)N + 4) + acc + dfb +
+                            (size_t)T.nnz * (4 + 4 * (size_t)N + 4) + 2 * acc + dfb +
                             (size_t)T.nnz * 16;
     if (!have_dev) fr = tot = 0;
     if (have_dev) {

@@ -207,7 +207,7 @@ int dyn_gpu_mttkrp_noalloc(dyn_gpu_ctx* ctx, int mode, int precision, float* Yha
  * atomic-free spMTTKRP launch per mode over that mode's sorted copy, all
  * reading the same factors.  With the linearised layout
  * (dyn_gpu_set_layout): one pass over the single Z-order copy that scatters
- * every mode's rows (FP32 atomics outside the hottest rows).  Yhat_out may be
+ * every mode's rows (FP64 atomics outside the hottest rows).  Yhat_out may be
  * NULL, or an array of N host pointers (entries may be NULL). */
 int dyn_gpu_mttkrp_all(dyn_gpu_ctx* ctx, int precision, float* const* Yhat_out,
                        int ld);
@@ -238,7 +238,7 @@ int dyn_gpu_build_linear(dyn_gpu_ctx* ctx, uint32_t* perm_out);
  * are rebuilt on demand after switching back).  On the linear layout
  * dyn_gpu_mttkrp and dyn_gpu_mttkrp_all run the all-modes pass (restricted
  * to the requested modes) with FP32 gathers whatever the precision argument
- * (the pass is bound by its scattered FP32 additions, not by its gathers),
+ * (the pass is bound by its scattered additions, not by its gathers),
  * and dyn_gpu_cpals runs Jacobi (a Gauss-Seidel request runs Jacobi with a
  * warning, as the reference's Morton path does, src/dynasor_cpals.cpp:
  * 232-240).  Results vary in the last bits from run to run (atomic

@@ -13,6 +13,9 @@ do not need an oracle run over the whole tensor:
     no clamped fit of 1.0 and a final fit equal to the FP64 true fit of the
     model it returns, or NaN where that fit is not finite (Jacobi diverges on
     this tensor, a property of the reference's update order);
+  * configuration 4 on the linearised (Z-order) copy: the single-pass
+    all-modes spMTTKRP against the per-mode copies', the column invariant,
+    and two Jacobi iterations against the per-mode path;
   * configuration 5 (1M x 100k x 2k, 200M uniform coordinates, Bernoulli
     labels from a planted rank-16 N(0, 0.5) model): 100 Adam steps at rank 32
     on the fused loss/adjoint, the final device loss against the FP64 oracle
@@ -130,6 +133,46 @@ def test_c4_full_remap_invariant_and_jacobi():
             continue
         fit = 1.0 - np.sqrt(max(r2, 0.0)) / np.sqrt(t2)
         assert abs(res.final_fit - fit) <= 1e-5 * max(1.0, abs(fit)), (res.final_fit, fit)
+
+
+def test_c4_full_linear_layout():
+    """The linearised copy of the 5-mode 400M tensor (72 index bits, so the key
+    drops the largest modes' lowest rank bits): its single-pass all-modes
+    spMTTKRP equals the per-mode copies' to FP32 summation-order rounding in
+    every mode, the column invariant holds, and Jacobi from the same initial
+    factors tracks the per-mode path."""
+    say("c4 linear: generate")
+    T, _, _ = host.generate((500_000, 100_000, 50_000, 5_000, 100), 400_000_000, seed=4,
+                            coords="zipf", alpha=1.0, values="planted", planted_rank=20,
+                            noise_frac=0.05, threads=16)
+    R = 32
+    rng = np.random.default_rng(0)
+    U = [rng.random((d, R), dtype=np.float32) for d in T.dims]
+    Y0 = init_factors(T.dims, R, 42, "normal")
+    with GpuContext(0) as ctx:
+        say("c4 linear: upload + Z-order copy")
+        ctx.upload(T)
+        ctx.set_factors(U, R)
+        ctx.set_layout(L.DYN_LAYOUT_LINEAR)
+        lin = ctx.mttkrp_all()
+        ctx.set_factors(Y0, R)
+        say("c4 linear: 2 Jacobi iterations")
+        jl = ctx.run_cpals(2, 0.0, kernel=L.DYN_CP_JACOBI)
+        say("c4 linear: per-mode copies")
+        ctx.set_layout(L.DYN_LAYOUT_COPIES)
+        ctx.set_factors(U, R)
+        cop = ctx.mttkrp_all()
+        ctx.set_factors(Y0, R)
+        jc = ctx.run_cpals(2, 0.0, kernel=L.DYN_CP_JACOBI)
+    for n in range(5):
+        a, b = lin[n].astype(np.float64), cop[n].astype(np.float64)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5, n
+    cols = [np.einsum("ir,ir->r", lin[n].astype(np.float64), U[n].astype(np.float64))
+            for n in range(5)]
+    for n in range(1, 5):
+        np.testing.assert_allclose(cols[n], cols[0], rtol=5e-5)
+    say("fits", list(jl.fit_history), list(jc.fit_history))
+    np.testing.assert_allclose(jl.fit_history, jc.fit_history, rtol=0, atol=1e-5)
 
 
 def test_c5_full_completion_adam_and_gradients():
