@@ -184,14 +184,16 @@ __device__ __forceinline__ float quad_sum(float x) {
 // The loss derivative and value of one nonzero, in FP32 (m itself is an FP32
 // sum); the values are summed in FP64.  The sigmoid takes the hardware
 // reciprocal (1 ulp) and the softplus its stable form max(m, 0) +
-// log1p(e^-|m|): an FP64 softplus per nonzero made the launch that sums the
-// loss 1.8x slower than the other two.
+// log(1 + e^-|m|) with the hardware exp and log (a few ulp; where 1 + e^-|m|
+// rounds to 1 the term is below 6e-8 and is dropped): an FP64 softplus per
+// nonzero made the launch that sums the loss 1.8x slower than the other two,
+// and the library log1pf still cost it three times the VALU of the others.
 __device__ __forceinline__ float dloss(int kind, float m, float y) {
     if (kind == DYN_LOSS_BERNOULLI) return __builtin_amdgcn_rcpf(1.0f + __expf(-m)) - y;
     return m - y;
 }
 __device__ __forceinline__ float lossval(int kind, float m, float y) {
-    if (kind == DYN_LOSS_BERNOULLI) return fmaxf(m, 0.0f) + log1pf(__expf(-fabsf(m))) - y * m;
+    if (kind == DYN_LOSS_BERNOULLI) return fmaxf(m, 0.0f) + __logf(1.0f + __expf(-fabsf(m))) - y * m;
     const float d = m - y;
     return 0.5f * d * d;
 }
