@@ -462,8 +462,9 @@ def main():
     if args.workload == "c4" and not args.no_extra and args.ooc is None and not linear:
         for name in ("c2", "c3"):
             w = WORKLOADS[name]
-            leg = cpals_leg(name, w, args.warmup, w["steps"], threads, False)
-            extra[name] = {"workload": leg["workload"], **summary(leg)}
+            leg = cpals_leg(name, w, args.warmup, w["steps"], threads, not args.no_cpu)
+            extra[name] = {"workload": leg["workload"], **summary(leg),
+                           "cpu_baseline": leg.get("cpu_baseline")}
         # the 3-mode tensor streamed from host memory in 5 chunks (Jacobi)
         w = dict(WORKLOADS["c2"], kernel="jacobi",
                  desc=WORKLOADS["c2"]["desc"].replace("Gauss-Seidel", "all-modes (Jacobi)") +
