@@ -26,8 +26,8 @@ LIB = PKG / "lib"
 BUILD = PKG / "_build"
 
 GPU_ARCH = "gfx950"
-HIP_SOURCES = ["dyn_ctx.hip", "dyn_remap.hip", "dyn_mttkrp.hip", "dyn_algebra.hip", "dyn_linear.hip",
-               "dyn_persist.hip", "dyn_stream.hip"]
+HIP_SOURCES = ["dyn_mttkrp.hip", "dyn_mttkrp_big.hip", "dyn_ctx.hip", "dyn_remap.hip", "dyn_algebra.hip",
+               "dyn_linear.hip", "dyn_persist.hip", "dyn_stream.hip"]
 HOST_SOURCES = ["dyn_io.cpp", "dyn_plan.cpp", "dyn_cpals_host.cpp", "dyn_gen.cpp", "dyn_dnp.cpp"]
 
 

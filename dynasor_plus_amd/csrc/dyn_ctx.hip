@@ -418,12 +418,6 @@ extern "C" int dyn_gpu_upload_coo(dyn_gpu_ctx* c, int N, uint64_t nnz,
     if (nnz >= 0xFFFFFFFFull)
         return set_err(c, DYN_ERR_INVALID, "nnz must be < 2^32 - 1, got %llu",
                        (unsigned long long)nnz);
-    // the spMTTKRP streams each column through one buffer resource with
-    // 32-bit byte offsets
-    if (nnz >= kMaxNnz)
-        return set_err(c, DYN_ERR_UNSUPPORTED,
-                       "nnz must be < 2^30 on one device (got %llu); shard larger tensors",
-                       (unsigned long long)nnz);
     if (!dims) return set_err(c, DYN_ERR_INVALID, "dims is null");
     if (nnz > 0 && (!vals || !idx))
         return set_err(c, DYN_ERR_INVALID, "vals/idx is null");

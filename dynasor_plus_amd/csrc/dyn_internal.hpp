@@ -97,7 +97,7 @@ constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a p
 constexpr uint32_t kPiece = 64 * kPieceLane;         // nonzeros per heavy piece (one wave)
 constexpr uint32_t kHeavyMin = 256;                  // longer rows are heavy
 constexpr uint32_t kLightWindow = 128;
-constexpr uint64_t kMaxNnz = 1ull << 30;             // nonzeros per device (4-byte columns < 4 GiB)
+constexpr uint64_t kMaxNnz = 0xFFFFFFFFull;          // nonzeros per device (32-bit positions)
 
 struct TimedPair {
     hipEvent_t a, b;

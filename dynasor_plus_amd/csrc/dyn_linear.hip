@@ -367,7 +367,7 @@ void free_linear(dyn_gpu_ctx* c) {
 int build_linear(dyn_gpu_ctx* c) {
     if (c->streamed) return reject_streamed(c, "the linearised layout");
     if (c->nnz >= kMaxNnz)
-        return set_err(c, DYN_ERR_UNSUPPORTED, "linearised layout: nnz %llu >= 2^30",
+        return set_err(c, DYN_ERR_UNSUPPORTED, "linearised layout: nnz %llu >= 2^32 - 1",
                        (unsigned long long)c->nnz);
     DYN_CHECK(ensure_relabel(c));
     free_linear(c);

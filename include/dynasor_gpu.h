@@ -103,9 +103,9 @@ int         dyn_gpu_device_memory(int device, uint64_t* free_bytes, uint64_t* to
 /* Copy a COO tensor (SoA: vals[nnz], idx[n][nnz]) to the device.  Replaces
  * any previous tensor and drops per-mode copies and factors.  Indices must
  * satisfy idx[n][k] < dims[n] (checked on the device; DYN_ERR_INVALID).
- * nnz must be below 2^30 (each 4-byte column under 4 GiB, the range of the
- * spMTTKRP's stream offsets); larger tensors return DYN_ERR_UNSUPPORTED and
- * are meant to be sharded over devices. */
+ * nnz must be below 2^32 - 1 (positions are 32-bit, as in the reference's
+ * .dnb format), else DYN_ERR_INVALID; columns of 4 GiB or more (2^30
+ * nonzeros or more) are read through per-piece buffer ranges. */
 int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
                        const uint32_t* dims, const float* vals,
                        const uint32_t* const* idx);

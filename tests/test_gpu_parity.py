@@ -329,10 +329,13 @@ def test_error_paths(gpu_ctx):
     with pytest.raises(L.DynGpuError) as e:
         gpu_ctx.mttkrp(0)
     assert e.value.code == L.DYN_ERR_STATE
-    # 2^30 nonzeros or more: rejected before any buffer is read
+    # 2^32 - 1 nonzeros or more (32-bit positions): rejected before any buffer
+    # is read; below that, null buffers are
     dims = (L._u32p._type_ * 3)(10, 10, 10)
+    rc = gpu_ctx.lib.dyn_gpu_upload_coo(gpu_ctx.h, 3, (1 << 32) - 1, dims, None, None)
+    assert rc == L.DYN_ERR_INVALID
     rc = gpu_ctx.lib.dyn_gpu_upload_coo(gpu_ctx.h, 3, 1 << 30, dims, None, None)
-    assert rc == L.DYN_ERR_UNSUPPORTED
+    assert rc == L.DYN_ERR_INVALID
 
 
 def test_memory_cap_reports_oom(monkeypatch):
