@@ -219,17 +219,23 @@ template <int RP, int NO, int KIND>
 constexpr int seg_threads() {
     return seg_regs<RP, NO, KIND>(1) <= 116 ? 1024 : (seg_regs<RP, NO, KIND>(1) <= 240 ? 512 : 256);
 }
-template <int RP, int NO, int KIND>
+// Nonzeros whose gathers are in flight together, counted with the raw row
+// words of the operand precision (BF16 rows take half the registers of FP32
+// ones: 5-mode rank-32 and 4-mode rank-64 BF16 launches go from 1 and 2 to 2
+// and 4 nonzeros in flight, 3 % and 5 % faster).
+template <int RP, int NO, typename FT, int KIND>
 constexpr int seg_hoist() {
     constexpr int budget = seg_threads<RP, NO, KIND>() == 1024 ? 128 : 240;
-    return seg_regs<RP, NO, KIND>(4) <= budget ? 4 : (seg_regs<RP, NO, KIND>(2) <= budget ? 2 : 1);
+    constexpr int rows = NO * Cols<RP, FT>::RD;
+    constexpr int rest = seg_regs<RP, NO, KIND>(0);
+    return rest + 4 * rows <= budget ? 4 : (rest + 2 * rows <= budget ? 2 : 1);
 }
 
 // Heavy pieces: rounds of 4 nonzeros per quad whose gathers are in flight
 // together -- two when the raw words of 8 nonzeros fit beside the rest.
 template <int RP, int NO, typename FT, int KIND>
 constexpr int seg_rounds() {
-    return (KIND == 0 && seg_hoist<RP, NO, KIND>() == 4 &&
+    return (KIND == 0 && seg_hoist<RP, NO, float, KIND>() == 4 &&
             seg_threads<RP, NO, KIND>() == 1024 &&
             8 * NO * Cols<RP, FT>::RD + seg_regs<RP, NO, KIND>(4) - 4 * NO * (RP / 4) <= 128)
                ? 2
@@ -482,7 +488,7 @@ template <int RP, int NO, typename FT, int KIND, bool BIG>
 __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a) {
     using C = Cols<RP, FT>;
     constexpr int CPL = C::CPL;
-    constexpr int HB = seg_hoist<RP, NO, KIND>();
+    constexpr int HB = seg_hoist<RP, NO, FT, KIND>();
     constexpr int TB = seg_threads<RP, NO, KIND>();
     constexpr int PL = (int)kPieceLane;
     constexpr int NR = seg_rounds<RP, NO, FT, KIND>();
