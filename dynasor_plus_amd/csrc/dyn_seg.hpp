@@ -365,9 +365,17 @@ __device__ __forceinline__ void accumulate(float* acc, float v, const float (*y)
     }
 }
 
-// Gathers of HB broadcast nonzeros (FP32: operand 0 reused from raw0prev when
-// the index repeats; BF16 rows are half as long and measured faster without
-// the reuse selects), then their accumulation in order.  `live(j)` says whether
+// Whether a quad reuses operand 0's row (the secondary sort mode's) from
+// registers while consecutive nonzeros share it.  FP32 rows always; BF16 rows
+// only when long (Rp >= 64) and with 3+ operands: at 4-mode rank 64 the reuse
+// is 3 % faster, at 5-mode rank 32 5 % slower and at 3-mode rank 32 2 %
+// slower than gathering every row (the selects cost more than the half-length
+// rows they save).
+template <int RP, int NO, typename FT>
+constexpr bool kReuse0() { return sizeof(FT) == 4 || (NO >= 3 && RP >= 64); }
+
+// Gathers of HB broadcast nonzeros (operand 0 reused from raw0prev when the
+// index repeats and kReuse0 holds), then their accumulation in order.  `live(j)` says whether
 // nonzero j of the batch exists; `before_step(j)` runs ahead of its
 // accumulation (row changes of light segments).
 template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, int SPLIT = 0,
@@ -386,7 +394,7 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT
         for (int h = 0; h < HB; ++h) {
             const int j = j0 + h;
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
-            reuse[h] = sizeof(FT) == 4 && ib[j][0] == before;
+            reuse[h] = kReuse0<RP, NO, FT>() && ib[j][0] == before;
             if (!reuse[h])
                 gather_op<RP, FT, SPLIT>(a, hot1, 0, gb.g[0], gb.l[0], ib[j][0], raw[h][0]);
 #pragma unroll
