@@ -77,6 +77,7 @@ struct SegArgs {
     double*         loss_part;    // [npieces + nseg] or null
     int             loss;
     uint32_t        nnz;          // nonzeros of the copy (value / index column length)
+    uint32_t        allhot;       // bit w: every row of operand w is staged in LDS
 };
 
 // ---------------------------------------------------------------------------
@@ -261,15 +262,6 @@ __device__ __forceinline__ void bcast_round(const float* sv4, const uint32_t (*s
     }
 }
 
-// A factor row by frequency rank (raw words of this lane): from the LDS copy
-// when hot, else from memory.  `gq` / `lq` are the operand's base pointers
-// already offset to this lane's first column.
-template <int RP, typename FT>
-__device__ __forceinline__ void gather_raw(const FT* gq, const FT* lq, uint32_t hot, uint32_t i,
-                                           uint32_t* r) {
-    load_raw<RP, FT>(((i < hot) ? lq : gq) + (size_t)i * RP, r);
-}
-
 // The same load from a known address space: LDS (ds_read) or global memory.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
@@ -295,12 +287,51 @@ __device__ __forceinline__ void load_raw_as(const FT* lane_row, uint32_t* r) {
     }
 }
 
+// A lane's part of a row from a 32-bit LDS byte address (ds_read).
+template <int RP, typename FT>
+__device__ __forceinline__ void load_raw_lds(uint32_t addr, uint32_t* r) {
+    using C = Cols<RP, FT>;
+    typedef __attribute__((address_space(3))) const u32x4_t v4p;
+    typedef __attribute__((address_space(3))) const u32x2_t v2p;
+    constexpr uint32_t LB = C::LW * sizeof(FT);   // bytes of one lane load
+#pragma unroll
+    for (int b = 0; b < C::NB; ++b) {
+        const uint32_t ab = addr + (uint32_t)b * 4u * LB;
+        if constexpr (LB == 16) {
+            const u32x4_t x = *(v4p*)(size_t)ab;
+            r[b * 4 + 0] = x.x;
+            r[b * 4 + 1] = x.y;
+            r[b * 4 + 2] = x.z;
+            r[b * 4 + 3] = x.w;
+        } else {
+            const u32x2_t x = *(v2p*)(size_t)ab;
+            r[b * 2 + 0] = x.x;
+            r[b * 2 + 1] = x.y;
+        }
+    }
+}
+
 // per-operand gather bases of one lane
 template <int NO, typename FT>
 struct GatherBase {
-    const FT* g[NO];
-    const FT* l[NO];
+    const FT* g[NO];        // rows in memory (this lane's first column)
+    const FT* l[NO];        // rows in LDS (generic addresses; the low word is the LDS address)
 };
+
+// A factor row by frequency rank (raw words of this lane): from the LDS copy
+// when hot, else from memory.  With AH, an operand whose rows all sit in LDS
+// (`all`, uniform over the launch) is read by ds_read at 32-bit addresses:
+// its rows then skip the vector-memory path and take no 64-bit address
+// arithmetic.  AH is on for BF16 rows up to 6 operands (FP32 rows and wider
+// launches spill with the second path).
+template <int RP, typename FT, bool AH, typename GB>
+__device__ __forceinline__ void gather_raw(const GB& gb, int w, uint32_t hot, bool all, uint32_t i,
+                                           uint32_t* r) {
+    if (AH && all)
+        load_raw_lds<RP, FT>((uint32_t)reinterpret_cast<uint64_t>(gb.l[w]) + i * (uint32_t)(RP * sizeof(FT)),
+                             r);
+    else load_raw<RP, FT>(((i < hot) ? gb.l[w] : gb.g[w]) + (size_t)i * RP, r);
+}
 
 // Whether heavy pieces take the hot-split path (3-mode tensors, both
 // operand precisions).
@@ -311,15 +342,15 @@ constexpr bool kSplitPath() { return NO == 2; }
 // SPLIT 1: a hot-split piece -- operand 1 comes from LDS when the piece is
 // hot (`hot1`, uniform over the wave) and from memory otherwise; operand 0
 // (reused across a fiber) from LDS or memory by rank; the others from memory.
-template <int RP, typename FT, int SPLIT>
-__device__ __forceinline__ void gather_op(const SegArgs& a, bool hot1, int w, const FT* gq,
-                                          const FT* lq, uint32_t i, uint32_t* r) {
+template <int RP, typename FT, int SPLIT, bool AH, typename GB>
+__device__ __forceinline__ void gather_op(const SegArgs& a, bool hot1, int w, const GB& gb,
+                                          uint32_t i, uint32_t* r) {
     if constexpr (SPLIT == 0) {
-        gather_raw<RP, FT>(gq, lq, a.hot[w], i, r);
+        gather_raw<RP, FT, AH>(gb, w, a.hot[w], (a.allhot >> w) & 1u, i, r);
     } else {
-        if (w == 0) gather_raw<RP, FT>(gq, lq, a.hot[0], i, r);
-        else if (w == 1 && hot1) load_raw_as<RP, FT, 3>(lq + (size_t)i * RP, r);
-        else load_raw_as<RP, FT, 1>(gq + (size_t)i * RP, r);
+        if (w == 0) gather_raw<RP, FT, AH>(gb, 0, a.hot[0], a.allhot & 1u, i, r);
+        else if (w == 1 && hot1) load_raw_as<RP, FT, 3>(gb.l[1] + (size_t)i * RP, r);
+        else load_raw_as<RP, FT, 1>(gb.g[w] + (size_t)i * RP, r);
     }
 }
 
@@ -379,13 +410,14 @@ constexpr bool kReuse0() { return sizeof(FT) == 4 || (NO >= 3 && RP >= 64); }
 // nonzero j of the batch exists; `before_step(j)` runs ahead of its
 // accumulation (row changes of light segments).
 template <int RP, int NO, typename FT, int KIND, int J, int HB, int CPL, int SPLIT = 0,
-          typename Live, typename Before>
-__device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT>& gb,
+          typename GB, typename Live, typename Before>
+__device__ __forceinline__ void batch4(const SegArgs& a, const GB& gb,
                                        const float* vb, const uint32_t (*ib)[NO], int q,
                                        float* acc, const float* arow, double* lsum,
                                        uint32_t& prev0, uint32_t* raw0prev, Live live,
                                        Before before_step, bool hot1 = false) {
     constexpr int RD = Cols<RP, FT>::RD;
+    constexpr bool AH = sizeof(FT) == 2 && NO <= 6;
 #pragma unroll
     for (int j0 = 0; j0 < J; j0 += HB) {
         uint32_t raw[HB][NO][RD];
@@ -396,10 +428,10 @@ __device__ __forceinline__ void batch4(const SegArgs& a, const GatherBase<NO, FT
             const uint32_t before = (j == 0) ? prev0 : ib[j - 1][0];
             reuse[h] = kReuse0<RP, NO, FT>() && ib[j][0] == before;
             if (!reuse[h])
-                gather_op<RP, FT, SPLIT>(a, hot1, 0, gb.g[0], gb.l[0], ib[j][0], raw[h][0]);
+                gather_op<RP, FT, SPLIT, AH>(a, hot1, 0, gb, ib[j][0], raw[h][0]);
 #pragma unroll
             for (int w = 1; w < NO; ++w)
-                gather_op<RP, FT, SPLIT>(a, hot1, w, gb.g[w], gb.l[w], ib[j][w], raw[h][w]);
+                gather_op<RP, FT, SPLIT, AH>(a, hot1, w, gb, ib[j][w], raw[h][w]);
         }
 #pragma unroll
         for (int h = 0; h < HB; ++h) {
@@ -441,8 +473,8 @@ __device__ __forceinline__ float qbf_at(float v, int t) {
 // before the accumulation.  Operand-1 rows thus hold no registers while the
 // gathers are in flight, and J = 6 memory gathers fit where the general
 // batch keeps 4 (plus 4 LDS rows).
-template <int RP, typename FT, int KIND, int S, int J, int CPL, typename Live>
-__device__ __forceinline__ void batch_hot(const SegArgs& a, const GatherBase<2, FT>& gb,
+template <int RP, typename FT, int KIND, int S, int J, int CPL, typename GB, typename Live>
+__device__ __forceinline__ void batch_hot(const SegArgs& a, const GB& gb,
                                           const float* sv4, const uint32_t (*si4)[4], int q,
                                           float* acc, const float* arow, double* lsum,
                                           uint32_t& prev0, uint32_t* raw0prev, Live live) {
@@ -455,7 +487,7 @@ __device__ __forceinline__ void batch_hot(const SegArgs& a, const GatherBase<2, 
         i0[j] = qb_at(si4[0][(S + j) % 4], (S + j) / 4);
         const uint32_t before = (j == 0) ? prev0 : i0[j - 1];
         reuse[j] = sizeof(FT) == 4 && i0[j] == before;
-        if (!reuse[j]) gather_raw<RP, FT>(gb.g[0], gb.l[0], a.hot[0], i0[j], raw0[j]);
+        if (!reuse[j]) gather_raw<RP, FT, false>(gb, 0, a.hot[0], false, i0[j], raw0[j]);
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -798,12 +830,30 @@ int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
             dims_o[0], (uint32_t)((budget - std::min(budget, (size_t)a.hot[1] * rowb)) / rowb));
         bytes = ((size_t)a.hot[0] + a.hot[1]) * rowb;
     } else {
-        const uint32_t per = (uint32_t)(budget / ((size_t)NO * rowb));
-        for (int w = 0; w < NO; ++w) {
-            a.hot[w] = std::min<uint32_t>(per, dims_o[w]);
-            bytes += (size_t)a.hot[w] * rowb;
+        // Operands whose rows all fit take them all, smallest first, up to
+        // half the budget (their gathers then never leave the LDS); the rest
+        // of the budget is shared out smallest first, each taking at most an
+        // equal share of what is left.
+        int ord[kMaxModes - 1];
+        for (int w = 0; w < NO; ++w) ord[w] = w;
+        std::sort(ord, ord + NO, [&](int x, int y) { return dims_o[x] < dims_o[y]; });
+        size_t whole = 0;
+        int k = 0;
+        for (; k < NO && whole + (size_t)dims_o[ord[k]] * rowb <= budget / 2; ++k) {
+            a.hot[ord[k]] = dims_o[ord[k]];
+            whole += (size_t)dims_o[ord[k]] * rowb;
         }
+        size_t left = budget - whole;
+        for (int m = k; m < NO; ++m) {
+            const size_t share = left / ((size_t)(NO - m) * rowb);
+            a.hot[ord[m]] = (uint32_t)std::min<size_t>(share, dims_o[ord[m]]);
+            left -= (size_t)a.hot[ord[m]] * rowb;
+        }
+        for (int w = 0; w < NO; ++w) bytes += (size_t)a.hot[w] * rowb;
     }
+    a.allhot = 0;
+    for (int w = 0; w < NO; ++w)
+        if (a.hot[w] >= dims_o[w]) a.allhot |= 1u << w;
     // a 3-mode launch reads the copy's hot-piece flags (the hot split)
     if (kSplitPath<NO, FT>() && !a.pc_hot)
         return set_err(c, DYN_ERR_STATE, "3-mode spMTTKRP launch without a hot split");
