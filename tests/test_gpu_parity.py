@@ -154,6 +154,40 @@ def test_mttkrp_bf16_operands(gpu_ctx, oracle, dims):
         assert frob_rel(got, oracle.mttkrp_f64(T, Y, n)) <= 2e-2
 
 
+@pytest.mark.parametrize("dims,R", [((3000, 2500, 2000, 40), 32),
+                                    ((2000, 1500, 1000, 300), 64),
+                                    ((4000, 3000, 800, 60, 25), 32)])
+def test_mttkrp_lds_whole_and_partial_operands(gpu_ctx, oracle, dims, R):
+    """Zipf-skewed coordinates (with repeats, which the MTTKRP sums) on 4- and
+    5-mode shapes whose small modes the launcher holds whole in LDS (BF16 rows
+    then read by ds_read) while the large ones are split between LDS ranks and
+    memory; rank 64 BF16 also reuses the secondary mode's row.  Both operand
+    precisions against the FP64 oracle (BF16 on BF16-rounded factors)."""
+    say("mttkrp lds whole/partial", dims, R)
+    rng = np.random.default_rng(41)
+    nnz = 200_000
+    idx = [np.minimum(rng.zipf(1.3, nnz) - 1, d - 1).astype(np.uint32) for d in dims]
+    for w, d in enumerate(dims):   # scatter the Zipf ranks over the mode
+        idx[w] = rng.permutation(d).astype(np.uint32)[idx[w]]
+    vals = rng.standard_normal(nnz).astype(np.float32)
+    T = Tensor(dims, vals, idx)
+    Y = [rng.standard_normal((d, R)).astype(np.float32) for d in dims]
+    gpu_ctx.upload(T)
+    gpu_ctx.set_factors(Y, R)
+
+    def to_bf16(a):
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        return (u.astype(np.uint32) << 16).view(np.float32)
+
+    Yb = [to_bf16(y) for y in Y]
+    for n in range(len(dims)):
+        got = gpu_ctx.mttkrp(n, L.DYN_PREC_FP32)
+        assert frob_rel(got, oracle.mttkrp_f64(T, Y, n)) <= 1e-5, n
+        got = gpu_ctx.mttkrp(n, L.DYN_PREC_BF16)
+        assert frob_rel(got, oracle.mttkrp_f64(T, Yb, n)) <= 1e-5, n
+
+
 def test_mttkrp_bitwise_reproducible(gpu_ctx):
     say("mttkrp reproducible")
     rng = np.random.default_rng(12)
