@@ -506,35 +506,60 @@ __global__ __launch_bounds__(kSolveBlock) void k_solve(const float* __restrict__
 #pragma unroll
     for (int c = 0; c < CL; ++c) sq[c] = 0.0;
 
-    // wave-uniform trip count, so the cross-lane reductions see all lanes
-    const uint32_t stride = gridDim.x * (kSolveBlock / 4);
-    for (uint32_t b = blockIdx.x * (kSolveBlock / 4) + (uint32_t)(wave * 16); b < rows; b += stride) {
-        const uint32_t i = b + (uint32_t)g;
-        const bool ok = i < rows;
-        double x[CL];
+    // NRW rows per quad share every W element read from LDS (the LDS reads,
+    // RP doubles per row and lane, bound the kernel at rank 64); each row's
+    // products are formed in the same order as with one row.  Wave-uniform
+    // trip count, so the cross-lane reductions see all lanes.
+    constexpr int NRW = (RP == 64) ? 2 : 1;
+    const uint32_t stride = gridDim.x * (kSolveBlock / 4) * NRW;
+    for (uint32_t b = (blockIdx.x * (kSolveBlock / 4) + (uint32_t)(wave * 16)) * NRW; b < rows;
+         b += stride) {
+        uint32_t i[NRW];
+        bool ok[NRW];
+        const float* yrow[NRW];
+        double x[NRW][CL];
 #pragma unroll
-        for (int c = 0; c < CL; ++c) x[c] = 0.0;
-        const float* yrow = Yhat + (size_t)(ok ? i : 0) * RP;
+        for (int r = 0; r < NRW; ++r) {
+            i[r] = b + (uint32_t)(r * 16) + (uint32_t)g;
+            ok[r] = i[r] < rows;
+            yrow[r] = Yhat + (size_t)(ok[r] ? i[r] : 0) * RP;
+#pragma unroll
+            for (int c = 0; c < CL; ++c) x[r][c] = 0.0;
+        }
         for (int r4 = 0; r4 < RP / 4; ++r4) {
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok) v = *reinterpret_cast<const float4*>(yrow + r4 * 4);
-            const double yv[4] = {(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+            double yv[NRW][4];
+#pragma unroll
+            for (int r = 0; r < NRW; ++r) {
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ok[r]) v = *reinterpret_cast<const float4*>(yrow[r] + r4 * 4);
+                yv[r][0] = (double)v.x;
+                yv[r][1] = (double)v.y;
+                yv[r][2] = (double)v.z;
+                yv[r][3] = (double)v.w;
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const double* wr = W + (r4 * 4 + k) * RP + q * CL;
 #pragma unroll
-                for (int c = 0; c < CL; ++c) x[c] = fma(yv[k], wr[c], x[c]);
+                for (int c = 0; c < CL; ++c) {
+                    const double wc = wr[c];
+#pragma unroll
+                    for (int r = 0; r < NRW; ++r) x[r][c] = fma(yv[r][k], wc, x[r][c]);
+                }
             }
         }
-        if (ok) {
 #pragma unroll
-            for (int c4 = 0; c4 < CL / 4; ++c4) {
-                float4 v = make_float4((float)x[c4 * 4 + 0], (float)x[c4 * 4 + 1],
-                                       (float)x[c4 * 4 + 2], (float)x[c4 * 4 + 3]);
-                *reinterpret_cast<float4*>(X + (size_t)i * RP + q * CL + c4 * 4) = v;
+        for (int r = 0; r < NRW; ++r) {
+            if (ok[r]) {
+#pragma unroll
+                for (int c4 = 0; c4 < CL / 4; ++c4) {
+                    float4 v = make_float4((float)x[r][c4 * 4 + 0], (float)x[r][c4 * 4 + 1],
+                                           (float)x[r][c4 * 4 + 2], (float)x[r][c4 * 4 + 3]);
+                    *reinterpret_cast<float4*>(X + (size_t)i[r] * RP + q * CL + c4 * 4) = v;
+                }
+#pragma unroll
+                for (int c = 0; c < CL; ++c) sq[c] = fma(x[r][c], x[r][c], sq[c]);
             }
-#pragma unroll
-            for (int c = 0; c < CL; ++c) sq[c] = fma(x[c], x[c], sq[c]);
         }
     }
     // column sums of x^2: over the 16 rows of the wave (lanes q, q+4, ...)
