@@ -974,22 +974,28 @@ extern "C" int dyn_gpu_cpals(dyn_gpu_ctx* c, const dyn_cp_opts* o, dyn_cp_result
         double* ipart = nullptr;
         uint32_t inwg = 0;
         for (int n = 0; n < N; ++n) {
-            // Grams current -> V_n^{-1} on the auxiliary stream
-            if (hipEventRecord(ev_g, c->stream) != hipSuccess ||
-                hipStreamWaitEvent(c->aux_stream, ev_g, 0) != hipSuccess)
-                return set_err(c, DYN_ERR_HIP, "event record/wait failed");
-            DYN_CHECK(chol(c, n, o->ridge, c->aux_stream));
-            if (hipEventRecord(ev_l, c->aux_stream) != hipSuccess)
-                return set_err(c, DYN_ERR_HIP, "event record failed");
-            if (!jacobi) {
+            if (jacobi) {
+                // nothing to overlap with (mode n's V needs mode n - 1's new
+                // Gram): V_n^{-1} in order on the engine stream, without the
+                // cross-stream hand-offs
+                DYN_CHECK(chol(c, n, o->ridge, c->stream));
+            } else {
+                // Grams current -> V_n^{-1} on the auxiliary stream while
+                // mode n's spMTTKRP runs
+                if (hipEventRecord(ev_g, c->stream) != hipSuccess ||
+                    hipStreamWaitEvent(c->aux_stream, ev_g, 0) != hipSuccess)
+                    return set_err(c, DYN_ERR_HIP, "event record/wait failed");
+                DYN_CHECK(chol(c, n, o->ridge, c->aux_stream));
+                if (hipEventRecord(ev_l, c->aux_stream) != hipSuccess)
+                    return set_err(c, DYN_ERR_HIP, "event record failed");
                 c->seg_pair = &E[1 + 2 * n];
                 const int mrc = mttkrp_n(n);
                 c->seg_pair = nullptr;
                 nev = n + 1;
                 DYN_CHECK(mrc);
+                if (hipStreamWaitEvent(c->stream, ev_l, 0) != hipSuccess)
+                    return set_err(c, DYN_ERR_HIP, "stream wait failed");
             }
-            if (hipStreamWaitEvent(c->stream, ev_l, 0) != hipSuccess)
-                return set_err(c, DYN_ERR_HIP, "stream wait failed");
             SolveParts sq;
             DYN_CHECK(solve(c, n, &sq));
             const bool last = (n == N - 1) && !jacobi;
