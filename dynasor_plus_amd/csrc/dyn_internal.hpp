@@ -67,6 +67,10 @@ struct ModeCopy {
     // hot_h come first (stable), and heavy pieces never straddle the split
     int       hot_w = -1;
     uint32_t  hot_h = 0;
+    // all-hot class split (4+ modes): inside every row of at least kClassMin
+    // nonzeros, the nonzeros whose every other-mode rank w is below cls_h[w]
+    // (in operand order) come first; pc_hot marks their pieces.  All zero: none.
+    uint32_t  cls_h[kMaxModes] = {0};
     uint32_t  nheavy = 0;
     uint32_t* heavy_row = nullptr;              // nheavy
     uint32_t* heavy_first = nullptr;            // nheavy : first piece
@@ -97,6 +101,7 @@ constexpr uint32_t kPieceLane = 4;                   // nonzeros per lane of a p
 constexpr uint32_t kPiece = 64 * kPieceLane;         // nonzeros per heavy piece (one wave)
 constexpr uint32_t kHeavyMin = 256;                  // longer rows are heavy
 constexpr uint32_t kLightWindow = 128;
+constexpr uint32_t kClassMin = 4096;                 // shortest row given an all-hot class
 constexpr uint64_t kMaxNnz = 0xFFFFFFFFull;          // nonzeros per device (32-bit positions)
 
 struct TimedPair {
@@ -253,6 +258,11 @@ void free_factors(dyn_gpu_ctx* c);
 // rebuilds its work tables with hot/cold pieces.  No-op when already split
 // for (w, H).
 int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H);
+// Reorders copy `mode` so that in every row of at least kClassMin nonzeros the
+// nonzeros whose operand-j rank is below H[j] for every other mode j (operands
+// in mode order) precede the others, stable within both, and rebuilds its work
+// tables with those pieces flagged in pc_hot.  No-op when already split at H.
+int ensure_class_split(dyn_gpu_ctx* c, int mode, const uint32_t* H);
 
 // Segmented spMTTKRP of mode `mode` from the per-mode copy into `out`
 // (I_mode x Rp floats, device).  `Yin[w]` are the gathered factors (FP32 or

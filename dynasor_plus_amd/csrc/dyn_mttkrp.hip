@@ -54,13 +54,37 @@ __global__ __launch_bounds__(kBlock) void k_heavy(const float* __restrict__ part
 // mode after the secondary sort mode) is split at hot_split_rows.  Tensors
 // with more modes (more operands than the split serves) measured faster on
 // the plain order with rank-ordered LDS rows of every operand read through
-// generic loads.
-static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp) {
+// generic loads; their BF16 launches add the all-hot class split (rows of
+// kClassMin+ nonzeros lead with the nonzeros whose every gathered row has an
+// LDS rank: those pieces read LDS only, by ds_read).  It measured 1.2 % faster
+// at 4-mode rank 64 and 0.3 % at 5-mode rank 32: the remaining memory
+// gathers still bound the launch.
+static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int prec) {
     if (c->N < 3) return DYN_OK;
     if (c->N > 3) {
-        // plain (row, secondary) order: undo a split
+        // plain (row, secondary) order: undo a 3-mode split; BF16 spMTTKRP on
+        // up to 7 modes and 192 gathered columns then splits off the nonzeros
+        // whose every row sits in LDS (an FP32 launch runs on such a copy as
+        // on the plain order)
         const ModeCopy& cp = c->copy[mode];
-        return cp.hot_w >= 0 ? remap_mode(c, mode, cp.secondary) : DYN_OK;
+        if (cp.hot_w >= 0) DYN_CHECK(remap_mode(c, mode, cp.secondary));
+        if (prec != DYN_PREC_BF16 || c->N > 7 || Rp * (c->N - 1) > 192 ||
+            (uint64_t)c->nnz * 4u > 0xFFFFFFF0ull)
+            return DYN_OK;
+        uint32_t dims_o[kMaxModes], H[kMaxModes] = {0};
+        int NO = 0;
+        for (int w = 0; w < c->N; ++w)
+            if (w != mode) dims_o[NO++] = c->dims[w];
+        plan_lds_rows(c, (size_t)Rp * sizeof(bf16), NO, dims_o, H);
+        bool same = true;
+        for (int j = 0; j < NO; ++j) {
+            if (H[j] == 0) return DYN_OK;
+            same = same && c->copy[mode].cls_h[j] == H[j];
+        }
+        if (same) return DYN_OK;
+        // split from the plain order, so each class keeps the secondary order
+        if (c->copy[mode].cls_h[0] != 0) DYN_CHECK(remap_mode(c, mode, cp.secondary));
+        return ensure_class_split(c, mode, H);
     }
     int others[kMaxModes], j = 0;
     for (int w = 0; w < c->N; ++w)
@@ -143,6 +167,7 @@ static int prepare_seg(dyn_gpu_ctx* c, int mode, int Rp, const void* const* Yin,
     a->pc_e = cp.pc_e;
     a->pc_row = cp.pc_row;
     a->pc_hot = cp.pc_hot;
+    for (int w = 0; w + 1 < kMaxModes; ++w) a->cls_h[w] = cp.cls_h[w];
     a->npieces = cp.npieces;
     a->empty_rows = cp.empty_rows;
     a->nempty = cp.nempty;
@@ -177,7 +202,7 @@ int mttkrp_mode(dyn_gpu_ctx* c, int mode, int prec, const void* const* Yin,
         if (!split_ready(c, mode, Rp))
             return set_err(c, DYN_ERR_STATE, "mode %d copy is not prepared for this launch", mode);
     } else {
-        DYN_CHECK(prepare_split(c, mode, Rp));
+        DYN_CHECK(prepare_split(c, mode, Rp, prec));
     }
     DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, false, part_buf));
     if (vals_override) {
@@ -443,7 +468,7 @@ int completion_grad(dyn_gpu_ctx* c, int loss, const float* const* A, int Rp,
         uint32_t dims_o[kMaxModes];
         const bool want_loss = (mode == 0 && loss_out);
         if (!c->copy[mode].built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
-        DYN_CHECK(prepare_split(c, mode, Rp));
+        DYN_CHECK(prepare_split(c, mode, Rp, DYN_PREC_FP32));
         DYN_CHECK(prepare_seg(c, mode, Rp, Yin, &a, &NO, dims_o, want_loss));
         a.out = grad[mode];
         a.An = A[mode];

@@ -564,11 +564,9 @@ __global__ void k_row_hot(const uint32_t* __restrict__ rp, const uint32_t* __res
 
 struct HotScatter {
     const uint32_t* rp;
-    const uint32_t* hs;
+    const uint32_t* hs;                     // exclusive scan of the hot flags (n + 1)
     uint32_t        dim;
     uint32_t        n;
-    uint32_t        H;
-    const uint32_t* key;                    // the split mode's index column
     const float*    vals_in;
     float*          vals_out;
     const uint32_t* perm_in;
@@ -593,30 +591,59 @@ __global__ void k_hot_scatter(HotScatter a) {
         const uint32_t hb = a.hs[b];
         const uint32_t nh = a.hs[e] - hb;
         const uint32_t hk = a.hs[k] - hb;
-        const uint32_t pos = (a.key[k] < a.H) ? b + hk : b + nh + (k - b - hk);
+        const uint32_t pos = (a.hs[k + 1] != a.hs[k]) ? b + hk : b + nh + (k - b - hk);
         a.vals_out[pos] = a.vals_in[k];
         a.perm_out[pos] = a.perm_in[k];
         for (int j = 0; j < a.ncols; ++j) a.idx_out[j][pos] = a.idx_in[j][k];
     }
 }
 
-int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
+// all-hot classes: flag a nonzero of a row of at least kClassMin nonzeros
+// when every other mode's rank is below its threshold
+struct ClassFlag {
+    const uint32_t* rp;
+    uint32_t        dim;
+    uint32_t        n;
+    const uint32_t* idx[kMaxModes];
+    uint32_t        H[kMaxModes];
+    int             nops;
+    uint32_t*       f;
+};
+
+__global__ void k_class_flag(ClassFlag a) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k64 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k64 <= a.n; k64 += stride) {
+        const uint32_t k = (uint32_t)k64;
+        uint32_t hot = 0;
+        if (k < a.n) {
+            uint32_t lo = 0, hi = a.dim;   // the last r with rp[r] <= k
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.rp[mid] <= k) lo = mid;
+                else hi = mid;
+            }
+            if (a.rp[lo + 1] - a.rp[lo] >= kClassMin) {
+                hot = 1;
+                for (int j = 0; j < a.nops; ++j) hot &= (a.idx[j][k] < a.H[j]) ? 1u : 0u;
+            }
+        }
+        a.f[k] = hot;
+    }
+}
+
+// Regroups copy `mode` inside every row: the nonzeros flagged in hs (n + 1
+// entries, flags on entry) first, in their order, then the rest; heavy
+// pieces are cut in both groups and interleaved in proportion (pc_hot: 1 for
+// a piece of flagged nonzeros).  Frees hs.
+static int regroup_rows(dyn_gpu_ctx* c, int mode, uint32_t* hs) {
     ModeCopy& cp = c->copy[mode];
-    if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
-    if (cp.hot_w == w && cp.hot_h == H) return DYN_OK;
     const uint32_t n = (uint32_t)c->nnz;
     const uint32_t dim = c->dims[mode];
-    // flags, exclusive scan over n + 1 entries (entry n = the hot total)
-    uint32_t* hs = nullptr;
     uint32_t* nh_dev = nullptr;
-    DYN_CHECK(dalloc_t(c, &hs, (size_t)n + 1));
     DYN_CHECK(dalloc_t(c, &nh_dev, std::max<uint32_t>(dim, 1)));
     void* sp2 = nullptr;
     const uint64_t nb = ((uint64_t)n + 1 + kScanTile - 1) / kScanTile;
     DYN_CHECK(scratch2(c, (nb + 16) * sizeof(uint32_t), &sp2));
-    hipLaunchKernelGGL(k_hot_flag, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
-                       cp.idx[w], n, H, hs);
-    DYN_CHECK(check_launch(c, "hot_flag"));
     DYN_CHECK(exclusive_scan(c, hs, (uint64_t)n + 1, (uint32_t*)sp2));
     // scatter every column into fresh arrays
     HotScatter a;
@@ -624,8 +651,6 @@ int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
     a.hs = hs;
     a.dim = dim;
     a.n = n;
-    a.H = H;
-    a.key = cp.idx[w];
     a.vals_in = cp.vals;
     a.perm_in = cp.perm;
     float* nv = nullptr;
@@ -670,9 +695,63 @@ int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
     }
     dfree(c, hs);
     dfree(c, nh_dev);
-    DYN_CHECK(build_segments(c, cp, rp, &nh));
+    return build_segments(c, cp, rp, &nh);
+}
+
+int ensure_class_split(dyn_gpu_ctx* c, int mode, const uint32_t* H) {
+    ModeCopy& cp = c->copy[mode];
+    if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
+    int nops = 0;
+    bool same = cp.hot_w < 0;
+    for (int v = 0; v < c->N; ++v) {
+        if (v == mode) continue;
+        same = same && cp.cls_h[nops] == H[nops];
+        ++nops;
+    }
+    if (same) return DYN_OK;
+    const uint32_t n = (uint32_t)c->nnz;
+    const uint32_t dim = c->dims[mode];
+    uint32_t* hs = nullptr;
+    DYN_CHECK(dalloc_t(c, &hs, (size_t)n + 1));
+    ClassFlag a;
+    a.rp = cp.rowptr;
+    a.dim = dim;
+    a.n = n;
+    a.nops = nops;
+    a.f = hs;
+    {
+        int j = 0;
+        for (int v = 0; v < c->N; ++v) {
+            if (v == mode) continue;
+            a.idx[j] = cp.idx[v];
+            a.H[j] = H[j];
+            ++j;
+        }
+    }
+    hipLaunchKernelGGL(k_class_flag, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream, a);
+    DYN_CHECK(check_launch(c, "class_flag"));
+    DYN_CHECK(regroup_rows(c, mode, hs));
+    for (int j = 0; j < kMaxModes; ++j) cp.cls_h[j] = j < nops ? H[j] : 0u;
+    cp.hot_w = -1;
+    cp.hot_h = 0;
+    return DYN_OK;
+}
+
+int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
+    ModeCopy& cp = c->copy[mode];
+    if (!cp.built) return set_err(c, DYN_ERR_STATE, "mode %d copy not built", mode);
+    if (cp.hot_w == w && cp.hot_h == H) return DYN_OK;
+    const uint32_t n = (uint32_t)c->nnz;
+    // flags over n + 1 entries (entry n: 0, so the scan's last entry is the total)
+    uint32_t* hs = nullptr;
+    DYN_CHECK(dalloc_t(c, &hs, (size_t)n + 1));
+    hipLaunchKernelGGL(k_hot_flag, dim3(grid_for((uint64_t)n + 1)), dim3(kBlock), 0, c->stream,
+                       cp.idx[w], n, H, hs);
+    DYN_CHECK(check_launch(c, "hot_flag"));
+    DYN_CHECK(regroup_rows(c, mode, hs));
     cp.hot_w = w;
     cp.hot_h = H;
+    for (int j = 0; j < kMaxModes; ++j) cp.cls_h[j] = 0;
     return DYN_OK;
 }
 

@@ -78,6 +78,8 @@ struct SegArgs {
     int             loss;
     uint32_t        nnz;          // nonzeros of the copy (value / index column length)
     uint32_t        allhot;       // bit w: every row of operand w is staged in LDS
+    uint32_t        cls_h[kMaxModes - 1];   // the copy's all-hot class thresholds (0: none)
+    uint32_t        cls_on;       // pc_hot marks pieces whose every gather is an LDS row
 };
 
 // ---------------------------------------------------------------------------
@@ -338,14 +340,27 @@ __device__ __forceinline__ void gather_raw(const GB& gb, int w, uint32_t hot, bo
 template <int NO, typename FT>
 constexpr bool kSplitPath() { return NO == 2; }
 
+// Whether a launch may run all-hot class pieces (every gather from LDS by
+// ds_read): BF16 spMTTKRP on 4 to 7 modes, ordinary column sizes, and at most
+// 192 gathered columns per nonzero (wider launches spill registers with both
+// piece variants in one kernel).
+template <int RP, int NO, typename FT, int KIND, bool BIG>
+constexpr bool kClassPath() {
+    return sizeof(FT) == 2 && KIND == 0 && NO >= 3 && NO <= 6 && RP * NO <= 192 && !BIG;
+}
+
 // Gathers of operand w.  SPLIT 0: no hot split (LDS or memory by rank).
 // SPLIT 1: a hot-split piece -- operand 1 comes from LDS when the piece is
 // hot (`hot1`, uniform over the wave) and from memory otherwise; operand 0
 // (reused across a fiber) from LDS or memory by rank; the others from memory.
+// SPLIT 2: an all-hot class piece -- every row from LDS at 32-bit addresses.
 template <int RP, typename FT, int SPLIT, bool AH, typename GB>
 __device__ __forceinline__ void gather_op(const SegArgs& a, bool hot1, int w, const GB& gb,
                                           uint32_t i, uint32_t* r) {
-    if constexpr (SPLIT == 0) {
+    if constexpr (SPLIT == 2) {
+        load_raw_lds<RP, FT>((uint32_t)reinterpret_cast<uint64_t>(gb.l[w]) + i * (uint32_t)(RP * sizeof(FT)),
+                             r);
+    } else if constexpr (SPLIT == 0) {
         gather_raw<RP, FT, AH>(gb, w, a.hot[w], (a.allhot >> w) & 1u, i, r);
     } else {
         if (w == 0) gather_raw<RP, FT, AH>(gb, 0, a.hot[0], a.allhot & 1u, i, r);
@@ -702,8 +717,14 @@ __global__ __launch_bounds__((seg_threads<RP, NO, KIND>())) void k_seg(SegArgs a
                 do_rounds(std::integral_constant<int, 2>{}, splitc);
                 if constexpr (NR == 1) do_rounds(std::integral_constant<int, 3>{}, splitc);
             };
-            // launches on three modes run on a hot-split copy
-            do_piece(std::integral_constant<int, kSplitPath<NO, FT>() ? 1 : 0>{});
+            // launches on three modes run on a hot-split copy; an all-hot
+            // class piece (wave-uniform flag) reads every row from LDS
+            if constexpr (kClassPath<RP, NO, FT, KIND, BIG>()) {
+                if (a.cls_on && a.pc_hot[p] != 0u) do_piece(std::integral_constant<int, 2>{});
+                else do_piece(std::integral_constant<int, 0>{});
+            } else {
+                do_piece(std::integral_constant<int, kSplitPath<NO, FT>() ? 1 : 0>{});
+            }
             const bool row_ends = (p + 1 == a.npieces) || (a.pc_row[p + 1] != row);
             const bool flush = row_ends || (p + 1 == p1);
             if (flush) {
@@ -815,9 +836,32 @@ inline uint32_t hot_split_rows(dyn_gpu_ctx* c, int Rp, uint32_t rows) {
     return (uint32_t)std::min<size_t>(rows, budget / ((size_t)Rp * 4));
 }
 
+// LDS rows of each operand on the plain order.  Operands whose rows all fit
+// take them all, smallest first, up to half the budget (their gathers then
+// never leave the LDS); the rest of the budget is shared out smallest first,
+// each taking at most an equal share of what is left.
+inline void plan_lds_rows(dyn_gpu_ctx* c, size_t rowb, int NO, const uint32_t* dims_o,
+                          uint32_t* hot) {
+    const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
+    int ord[kMaxModes - 1];
+    for (int w = 0; w < NO; ++w) ord[w] = w;
+    std::sort(ord, ord + NO, [&](int x, int y) { return dims_o[x] < dims_o[y]; });
+    size_t whole = 0;
+    int k = 0;
+    for (; k < NO && whole + (size_t)dims_o[ord[k]] * rowb <= budget / 2; ++k) {
+        hot[ord[k]] = dims_o[ord[k]];
+        whole += (size_t)dims_o[ord[k]] * rowb;
+    }
+    size_t left = budget - whole;
+    for (int m = k; m < NO; ++m) {
+        const size_t share = left / ((size_t)(NO - m) * rowb);
+        hot[ord[m]] = (uint32_t)std::min<size_t>(share, dims_o[ord[m]]);
+        left -= (size_t)hot[ord[m]] * rowb;
+    }
+}
+
 template <int RP, int NO, typename FT, int KIND, bool BIG>
 int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
-    // hot rows: equal share of the LDS budget per gathered operand
     const size_t budget = std::min<size_t>(c->max_lds, 160u * 1024u) - 1024u;
     const size_t rowb = (size_t)RP * sizeof(FT);
     size_t bytes = 0;
@@ -830,26 +874,15 @@ int launch_seg(dyn_gpu_ctx* c, SegArgs& a, const uint32_t* dims_o) {
             dims_o[0], (uint32_t)((budget - std::min(budget, (size_t)a.hot[1] * rowb)) / rowb));
         bytes = ((size_t)a.hot[0] + a.hot[1]) * rowb;
     } else {
-        // Operands whose rows all fit take them all, smallest first, up to
-        // half the budget (their gathers then never leave the LDS); the rest
-        // of the budget is shared out smallest first, each taking at most an
-        // equal share of what is left.
-        int ord[kMaxModes - 1];
-        for (int w = 0; w < NO; ++w) ord[w] = w;
-        std::sort(ord, ord + NO, [&](int x, int y) { return dims_o[x] < dims_o[y]; });
-        size_t whole = 0;
-        int k = 0;
-        for (; k < NO && whole + (size_t)dims_o[ord[k]] * rowb <= budget / 2; ++k) {
-            a.hot[ord[k]] = dims_o[ord[k]];
-            whole += (size_t)dims_o[ord[k]] * rowb;
-        }
-        size_t left = budget - whole;
-        for (int m = k; m < NO; ++m) {
-            const size_t share = left / ((size_t)(NO - m) * rowb);
-            a.hot[ord[m]] = (uint32_t)std::min<size_t>(share, dims_o[ord[m]]);
-            left -= (size_t)a.hot[ord[m]] * rowb;
-        }
+        plan_lds_rows(c, rowb, NO, dims_o, a.hot);
         for (int w = 0; w < NO; ++w) bytes += (size_t)a.hot[w] * rowb;
+    }
+    // all-hot class pieces need the copy split at exactly these LDS rows
+    a.cls_on = 0;
+    if (kClassPath<RP, NO, FT, KIND, BIG>() && a.pc_hot) {
+        a.cls_on = 1;
+        for (int w = 0; w < NO; ++w)
+            if (a.cls_h[w] == 0 || a.cls_h[w] != a.hot[w]) a.cls_on = 0;
     }
     a.allhot = 0;
     for (int w = 0; w < NO; ++w)

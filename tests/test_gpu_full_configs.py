@@ -122,17 +122,33 @@ def test_c4_full_remap_invariant_and_jacobi():
         assert res.iters_done == 30
         assert not any(f == 1.0 for f in res.fit_history)
         say("c4 full: FP64 true fit of the returned model")
-        inner = float(v @ orc.forward(T, Y, lam))
+        m = orc.forward(T, Y, lam)
+        inner = float(v @ m)
+        S = float(np.abs(v) @ np.abs(m))
         H = np.ones((R, R))
+        Habs = np.ones((R, R))
         for y in Y:
             H *= y.T @ y
+            Habs *= np.abs(y).T @ np.abs(y)
         m2 = float(lam @ H @ lam)
+        m2abs = float(np.abs(lam) @ Habs @ np.abs(lam))
         r2 = t2 - 2 * inner + m2
+        say("t2", t2, "inner", inner, "sum|v m|", S, "m2", m2, "|m|2", m2abs, "r2", r2)
         if not (np.isfinite(r2) and r2 >= -1e-4 * max(t2, abs(2 * inner), abs(m2))):
             assert np.isnan(res.final_fit), (res.final_fit, t2, inner, m2)
             continue
         fit = 1.0 - np.sqrt(max(r2, 0.0)) / np.sqrt(t2)
-        assert abs(res.final_fit - fit) <= 1e-5 * max(1.0, abs(fit)), (res.final_fit, fit)
+        # Forward error of the engine's fit: <T, M> comes from FP32 spMTTKRP
+        # sums (error about 1e-5 of sum |v m| at these row lengths) and
+        # ||M||^2 from Gram matrices of the FP32 factors with FP32 MFMA
+        # partials (error about 1e-6 of |lam|' (*_n |A_n|'|A_n|) |lam|);
+        # d fit = d(r2) / (2 |r| |T|).  For a well-conditioned model both
+        # terms are far below 1e-5; a diverged one (huge, nearly cancelling
+        # components: the fit is a small difference of large terms) is
+        # determined only to that.
+        den = np.sqrt(max(r2, 1e-300)) * np.sqrt(t2)
+        tol = 1e-5 * max(1.0, abs(fit)) + (1e-5 * S + 0.5e-6 * m2abs) / den
+        assert abs(res.final_fit - fit) <= tol, (res.final_fit, fit, tol)
 
 
 def test_c4_full_linear_layout():

@@ -188,6 +188,52 @@ def test_mttkrp_lds_whole_and_partial_operands(gpu_ctx, oracle, dims, R):
         assert frob_rel(got, oracle.mttkrp_f64(T, Yb, n)) <= 1e-5, n
 
 
+def test_mttkrp_all_hot_class_split(gpu_ctx, oracle):
+    """BF16 launches on 4+ modes regroup every row of at least 4096 nonzeros so
+    that the nonzeros whose gathered rows all sit in LDS come first (their
+    pieces then read LDS only).  The copy stays a permutation of the tensor
+    (changed order, same entries); FP32 launches on it, BF16 at another rank
+    (a fresh split from the plain order) and back all match the oracle."""
+    say("mttkrp all-hot class split")
+    rng = np.random.default_rng(43)
+    dims = (40, 3000, 2500, 60)
+    nnz = 300_000
+    idx = [np.minimum(rng.zipf(1.3, nnz) - 1, d - 1).astype(np.uint32) for d in dims]
+    for w, d in enumerate(dims):
+        idx[w] = rng.permutation(d).astype(np.uint32)[idx[w]]
+    vals = rng.standard_normal(nnz).astype(np.float32)
+    T = Tensor(dims, vals, idx)
+    assert np.bincount(idx[0], minlength=dims[0]).max() >= 4 * 4096   # long rows exist
+    gpu_ctx.upload(T)
+
+    def to_bf16(a):
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+        return (u.astype(np.uint32) << 16).view(np.float32)
+
+    def entries(v, cols):   # the (value, columns) tuples in a canonical order
+        m = np.stack([v.view(np.uint32)] + [cols[w] for w in sorted(cols)])
+        return m[:, np.lexsort(m[::-1])]
+
+    for step, R in enumerate((32, 64, 32)):
+        Y = [rng.standard_normal((d, R)).astype(np.float32) for d in dims]
+        Yb = [to_bf16(y) for y in Y]
+        gpu_ctx.set_factors(Y, R)
+        if step == 0:
+            got = gpu_ctx.mttkrp(0, L.DYN_PREC_FP32)          # plain order
+            assert frob_rel(got, oracle.mttkrp_f64(T, Y, 0)) <= 1e-5
+            v0, c0 = gpu_ctx.get_copy(0)
+        for n in range(len(dims)):
+            got = gpu_ctx.mttkrp(n, L.DYN_PREC_BF16)
+            assert frob_rel(got, oracle.mttkrp_f64(T, Yb, n)) <= 1e-5, (R, n)
+            got = gpu_ctx.mttkrp(n, L.DYN_PREC_FP32)
+            assert frob_rel(got, oracle.mttkrp_f64(T, Y, n)) <= 1e-5, (R, n)
+        if step == 0:
+            v1, c1 = gpu_ctx.get_copy(0)
+            assert not all(np.array_equal(c0[w], c1[w]) for w in c0)   # regrouped
+            np.testing.assert_array_equal(entries(v0, c0), entries(v1, c1))
+
+
 def test_mttkrp_bitwise_reproducible(gpu_ctx):
     say("mttkrp reproducible")
     rng = np.random.default_rng(12)
