@@ -68,7 +68,9 @@ static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int prec) {
         // on the plain order)
         const ModeCopy& cp = c->copy[mode];
         if (cp.hot_w >= 0) DYN_CHECK(remap_mode(c, mode, cp.secondary));
-        if (prec != DYN_PREC_BF16 || c->N > 7 || Rp * (c->N - 1) > 192 ||
+        // (not on streamed chunks: a chunk's copy is rebuilt every sweep, and
+        // the regroup would cost more than its launch saves)
+        if (prec != DYN_PREC_BF16 || c->streamed || c->N > 7 || Rp * (c->N - 1) > 192 ||
             (uint64_t)c->nnz * 4u > 0xFFFFFFF0ull)
             return DYN_OK;
         uint32_t dims_o[kMaxModes], H[kMaxModes] = {0};
