@@ -483,7 +483,8 @@ extern "C" int dyn_gpu_remap(dyn_gpu_ctx* c, int mode, int secondary,
     if (secondary >= c->N || secondary == mode)
         return set_err(c, DYN_ERR_INVALID, "bad secondary mode %d", secondary);
     DYN_HIP(c, hipSetDevice(c->device));
-    DYN_CHECK(remap_mode(c, mode, secondary));
+    // the reference's order (secondary by input index), for comparison
+    DYN_CHECK(remap_mode(c, mode, secondary, false));
     const ModeCopy& cp = c->copy[mode];
     if (rowptr_out) {
         std::vector<uint32_t> tmp(c->dims[mode] + 1);

@@ -49,6 +49,7 @@ inline int padded_rank(int R) {
 struct ModeCopy {
     bool      built = false;
     int       secondary = -1;
+    bool      rank_key = false;                 // secondary key: rank (true) or input index
     float*    vals = nullptr;                   // nnz
     uint32_t* idx[kMaxModes] = {nullptr};      // relab[w][idx[w]] for w != mode
     uint32_t* rowptr = nullptr;                 // I_mode + 1
@@ -241,7 +242,14 @@ int check_launch(dyn_gpu_ctx* c, const char* what);
 void lds_limit_once(dyn_gpu_ctx* c, std::atomic<uint64_t>* done, const void* fn, int bytes);
 
 // building blocks implemented in the .hip units -----------------------------
-int remap_mode(dyn_gpu_ctx* c, int mode, int secondary);
+// Copy `mode` sorted by (index of `mode`, secondary key), where the secondary
+// key is the secondary mode's input index (the reference's order) unless
+// `engine_order` is set and the secondary mode's 1024 most frequent indices
+// hold under a quarter of the nonzeros (nearly uniform coordinates): then it
+// is the secondary mode's frequency rank, the order in which its factor rows
+// sit in memory, so consecutive nonzeros of a row gather neighbouring rows.
+// secondary < 0: stable by input position.
+int remap_mode(dyn_gpu_ctx* c, int mode, int secondary, bool engine_order = true);
 // Columns and row pointer of copy `mode` from cp.perm and the sorted keys.
 int materialize_copy(dyn_gpu_ctx* c, int mode, const uint32_t* keys);
 // Work tables from the row pointer (checked monotone and spanning nnz).

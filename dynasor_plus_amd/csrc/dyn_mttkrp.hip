@@ -67,7 +67,7 @@ static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int prec) {
         // whose every row sits in LDS (an FP32 launch runs on such a copy as
         // on the plain order)
         const ModeCopy& cp = c->copy[mode];
-        if (cp.hot_w >= 0) DYN_CHECK(remap_mode(c, mode, cp.secondary));
+        if (cp.hot_w >= 0) DYN_CHECK(remap_mode(c, mode, cp.secondary, cp.rank_key));
         // (not on streamed chunks: a chunk's copy is rebuilt every sweep, and
         // the regroup would cost more than its launch saves)
         if (prec != DYN_PREC_BF16 || c->streamed || c->N > 7 || Rp * (c->N - 1) > 192 ||
@@ -85,7 +85,8 @@ static int prepare_split(dyn_gpu_ctx* c, int mode, int Rp, int prec) {
         }
         if (same) return DYN_OK;
         // split from the plain order, so each class keeps the secondary order
-        if (c->copy[mode].cls_h[0] != 0) DYN_CHECK(remap_mode(c, mode, cp.secondary));
+        if (c->copy[mode].cls_h[0] != 0)
+            DYN_CHECK(remap_mode(c, mode, cp.secondary, cp.rank_key));
         return ensure_class_split(c, mode, H);
     }
     int others[kMaxModes], j = 0;
