@@ -244,11 +244,11 @@ void lds_limit_once(dyn_gpu_ctx* c, std::atomic<uint64_t>* done, const void* fn,
 // building blocks implemented in the .hip units -----------------------------
 // Copy `mode` sorted by (index of `mode`, secondary key), where the secondary
 // key is the secondary mode's input index (the reference's order) unless
-// `engine_order` is set and the secondary mode's 1024 most frequent indices
-// hold under a quarter of the nonzeros (nearly uniform coordinates): then it
-// is the secondary mode's frequency rank, the order in which its factor rows
-// sit in memory, so consecutive nonzeros of a row gather neighbouring rows.
-// secondary < 0: stable by input position.
+// `engine_order` is set and that mode has more than 2048 rows: then it is the
+// secondary mode's frequency rank (the order in which its factor rows sit in
+// memory, so consecutive nonzeros of a row gather neighbouring rows), with
+// the 1024 most frequent ranks hashed over the range so that the LDS-resident
+// rows spread evenly along each row.  secondary < 0: stable by input position.
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary, bool engine_order = true);
 // Columns and row pointer of copy `mode` from cp.perm and the sorted keys.
 int materialize_copy(dyn_gpu_ctx* c, int mode, const uint32_t* keys);

@@ -755,17 +755,29 @@ int ensure_hot_split(dyn_gpu_ctx* c, int mode, int w, uint32_t H) {
     return DYN_OK;
 }
 
-// Whether the engine's copies key the secondary sort by frequency rank: only
-// where no few rows take most gathers.  Measured on one box, all copies
-// rank-keyed: 3 % faster on a uniform 3-mode completion tensor (1M x 100k x
-// 2k, 200M nonzeros), but 1.6 % slower on a 4-mode Zipf(1.2) tensor, whose
-// rank order puts each row's LDS-resident secondary rows first, and within
-// 0.4 % on the 5-mode Zipf(1.0) one.
+// The engine's copies key the secondary sort by frequency rank (the memory
+// order of the secondary operand's rows, so consecutive nonzeros of a row
+// gather neighbouring rows), except that the 1024 most frequent ranks, whose
+// rows the launches hold in LDS, are hashed over the rank range: sorted
+// first they would leave some waves' pieces all-LDS and others all-memory.
+// Measured on one box against the input-index key: 5-mode Zipf(1.0) BF16
+// Jacobi 1.3 % faster, 4-mode Zipf(1.2) 0.7 %, uniform completion 3 %
+// (plain rank keys: 1.6 % slower on the Zipf(1.2) tensor).  Modes of up to
+// 2048 rows keep the input-index key.
 static bool secondary_rank_key(const dyn_gpu_ctx* c, int secondary) {
-    const std::vector<uint32_t>& rc = c->rank_count_host[secondary];
-    uint64_t top = 0;
-    for (size_t r = 0; r < rc.size() && r < 1024; ++r) top += rc[r];
-    return top * 4 < c->nnz;
+    return c->dims[secondary] > 2048;
+}
+
+// key = rank for ranks >= 1024, else 1024 + a multiplicative hash of the rank
+// scaled to [0, dim - 1024): always below dim
+__global__ void k_spread_key(const uint32_t* __restrict__ relab, const uint32_t* __restrict__ ix,
+                             uint32_t n, uint32_t span, uint32_t* __restrict__ key) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const uint32_t r = relab[ix[k]];
+        key[k] = r >= 1024u ? r
+                            : 1024u + (uint32_t)(((uint64_t)(r * 0x9E3779B1u) * span) >> 32);
+    }
 }
 
 int remap_mode(dyn_gpu_ctx* c, int mode, int secondary, bool engine_order) {
@@ -803,8 +815,9 @@ int remap_mode(dyn_gpu_ctx* c, int mode, int secondary, bool engine_order) {
         hipLaunchKernelGGL(k_iota, dim3(g), dim3(kBlock), 0, c->stream, vA, n);
         if (secondary >= 0) {
             if (rank_key) {
-                hipLaunchKernelGGL(k_gather_u32, dim3(g), dim3(kBlock), 0, c->stream,
-                                   c->relab[secondary], c->idx[secondary], kA, n);
+                hipLaunchKernelGGL(k_spread_key, dim3(g), dim3(kBlock), 0, c->stream,
+                                   c->relab[secondary], c->idx[secondary], n,
+                                   c->dims[secondary] - 1024u, kA);
                 DYN_CHECK(check_launch(c, "remap rank key"));
             } else {
                 DYN_HIP(c, hipMemcpyAsync(kA, c->idx[secondary], (size_t)n * 4,
