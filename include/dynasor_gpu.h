@@ -119,10 +119,17 @@ int dyn_gpu_upload_coo(dyn_gpu_ctx* ctx, int num_modes, uint64_t nnz,
  * idx[secondary] (composite key); -1 keeps input order within rows.
  * spMTTKRPs on 3-mode tensors (FP32 or BF16 operands) later reorder the
  * nonzeros inside each row (hot split: those whose next-mode factor row is
- * held in LDS first, both classes stable); the row pointer is unchanged. */
+ * held in LDS first, both classes stable), and BF16 spMTTKRPs on 4 to 7
+ * modes those of rows of 4096+ nonzeros (the nonzeros whose every gathered
+ * row is held in LDS first); the row pointer is unchanged. */
 int dyn_gpu_remap(dyn_gpu_ctx* ctx, int mode, int secondary,
                   uint64_t* rowptr_out, uint32_t* perm_out);
-/* Build every mode's copy (secondary key: the first other mode). */
+/* Build every mode's copy that is not built yet, secondary mode = the first
+ * other mode.  Unlike dyn_gpu_remap, whose secondary key is the input index,
+ * these copies key the secondary sort by that mode's frequency rank (the
+ * memory order of its factor rows) when it has more than 2048 rows, with its
+ * 1024 most frequent ranks hashed over the range; rows and results are the
+ * same, the order within rows differs. */
 int dyn_gpu_remap_all(dyn_gpu_ctx* ctx);
 /* Copy the mode-`mode` sorted copy back, in its current order (see the hot
  * split above): vals_out[nnz] and, for every mode w != mode, idx_out[w][nnz]
