@@ -247,6 +247,39 @@ def test_mttkrp_bitwise_reproducible(gpu_ctx):
         np.testing.assert_array_equal(x, y)
 
 
+def test_engine_copy_order(gpu_ctx, oracle):
+    """dyn_gpu_remap_all's copies order each row by the secondary mode's
+    frequency rank (most nonzeros first, ties by index) except for its 1024
+    most frequent ranks, which are scattered; dyn_gpu_remap keeps the
+    reference's order (secondary input index).  Both hold the same rows."""
+    say("engine copy order")
+    rng = np.random.default_rng(44)
+    dims = (50, 6000, 300)
+    nnz = 120_000
+    idx = [np.minimum(rng.zipf(1.2, nnz) - 1, d - 1).astype(np.uint32) for d in dims]
+    for w, d in enumerate(dims):
+        idx[w] = rng.permutation(d).astype(np.uint32)[idx[w]]
+    T = Tensor(dims, rng.standard_normal(nnz).astype(np.float32), idx)
+    cnt = np.bincount(idx[1], minlength=dims[1])
+    order = np.lexsort((np.arange(dims[1]), -cnt))       # rank -> index
+    rank = np.empty(dims[1], np.int64)
+    rank[order] = np.arange(dims[1])
+    gpu_ctx.upload(T)
+    gpu_ctx.remap_all()                                  # copy 0: secondary mode 1
+    v_e, c_e = gpu_ctx.get_copy(0)
+    rowptr, perm = gpu_ctx.remap(0, 1)                   # the reference's order
+    operm, orow = oracle.remap(T, 0, 1)
+    np.testing.assert_array_equal(perm, operm)
+    v_r, c_r = gpu_ctx.get_copy(0)
+    assert not np.array_equal(c_e[1], c_r[1])
+    for r in range(dims[0]):
+        b, e = int(rowptr[r]), int(rowptr[r + 1])
+        np.testing.assert_array_equal(np.sort(c_e[1][b:e]), np.sort(c_r[1][b:e]))
+        rk = rank[c_e[1][b:e]]
+        cold = rk[rk >= 1024]
+        assert np.all(np.diff(cold) >= 0), r
+
+
 @pytest.mark.parametrize("secondary", [-1, 1])
 def test_remap_bit_exact(gpu_ctx, oracle, secondary):
     say("remap", secondary)
